@@ -1,0 +1,160 @@
+/*
+ * qsv.h -- C ABI of the MI355X state-vector engine (libqclabpp_amd.so).
+ *
+ * This is the drop-in boundary for QCLAB++'s simulation hot path:
+ *   QCircuit<T>::simulate / apply(Op, nbQubits, std::vector<T>&, offset)
+ *     (include/qclab/QCircuit.hpp:130-143,164-166)
+ *   -> QObject<T>::apply (include/qclab/QObject.hpp:52-53)
+ *   -> per-gate sweeps apply2/apply2b/apply4/apply4b/apply4bc
+ *     (src/qgates/apply.hpp:258-371), and their OpenMP-offload twins
+ *     apply_device* / simulate_device (src/qgates/apply.hpp:373-532,
+ *     include/qclab/QCircuit.hpp:145-173).
+ *
+ * Conventions (identical to the reference):
+ *   - qubit 0 is the most significant bit: qubit q has amplitude stride
+ *     2^(n-1-q)                                    (src/qgates/apply.hpp:264)
+ *   - rotations carry (cos(theta/2), sin(theta/2)); Phase/CPhase carry
+ *     (cos(theta), sin(theta))      (include/qclab/QRotation.hpp:44-46,
+ *                                    include/qclab/qgates/Phase.hpp:51-55)
+ *   - Op 'N','T','C' applies M, M^T, M^H of the gate's matrix
+ *                                    (include/qclab/dense/transpose.hpp:12-35)
+ *   - generic matrices are column-major complex (re,im interleaved),
+ *     element (i,j) at [2*(i + j*size)]    (include/qclab/dense/SquareMatrix.hpp:125-132)
+ *   - 2-qubit basis order is 2*bit(q0)+bit(q1) for q0 < q1
+ *                                    (src/qgates/apply.hpp:289-308)
+ *
+ * Errors: every function returns an int status (QSV_OK == 0); no exception
+ * crosses this boundary.  qsv_last_error() returns a thread-local message
+ * for the last failing call on the calling thread.
+ *
+ * Streams: entry points that take a `stream` argument enqueue their work on
+ * that HIP stream (NULL = the default stream) and return once it is
+ * enqueued, except where a host result (a number, a host buffer) is
+ * returned: those synchronise the stream first.
+ */
+#ifndef QSV_H
+#define QSV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define QSV_OK        0
+#define QSV_EINVAL    1  /* bad argument: null pointer, size, op code      */
+#define QSV_EQUBIT    2  /* qubit index out of range or repeated           */
+#define QSV_ENOMEM    3  /* device or host allocation failed               */
+#define QSV_EHIP      4  /* a HIP runtime call failed                      */
+#define QSV_EDTYPE    5  /* unsupported element type                       */
+#define QSV_ENODEV    6  /* no usable GPU                                  */
+#define QSV_EUNSUP    7  /* valid request this build does not implement    */
+
+/* ---- element types ----------------------------------------------------- */
+#define QSV_C64   1   /* std::complex<float>  : 8 bytes per amplitude  */
+#define QSV_C128  2   /* std::complex<double> : 16 bytes per amplitude */
+
+/* ---- Op (include/qclab/util.hpp:15-20) ---------------------------------- */
+#define QSV_OP_N  'N'
+#define QSV_OP_T  'T'
+#define QSV_OP_C  'C'
+
+/* ---- gate kinds: one per QCLAB++ gate class ------------------------------ */
+enum qsv_gate_kind {
+  QSV_GATE_I     = 0,   /* Identity        include/qclab/qgates/Identity.hpp     */
+  QSV_GATE_H     = 1,   /* Hadamard        Hadamard.hpp:46-51                    */
+  QSV_GATE_X     = 2,   /* PauliX          PauliX.hpp:46-49                      */
+  QSV_GATE_Y     = 3,   /* PauliY          PauliY.hpp:46-49                      */
+  QSV_GATE_Z     = 4,   /* PauliZ          PauliZ.hpp:46-49                      */
+  QSV_GATE_RX    = 5,   /* RotationX       RotationX.hpp:104-108                 */
+  QSV_GATE_RY    = 6,   /* RotationY       RotationY.hpp:104-107                 */
+  QSV_GATE_RZ    = 7,   /* RotationZ       RotationZ.hpp:104-108                 */
+  QSV_GATE_P     = 8,   /* Phase           Phase.hpp:119-122                     */
+  QSV_GATE_T     = 9,   /* Phase45         Phase45.hpp:46-51                     */
+  QSV_GATE_S     = 10,  /* Phase90         Phase90.hpp:46-49                     */
+  QSV_GATE_U1    = 11,  /* QGate1 / PointerGate1: dense 2x2 from mat[]           */
+  QSV_GATE_CX    = 12,  /* CX / CNOT       CX.hpp, CNOT.hpp:11-12                */
+  QSV_GATE_CY    = 13,  /* CY                                                    */
+  QSV_GATE_CZ    = 14,  /* CZ                                                    */
+  QSV_GATE_CP    = 15,  /* CPhase          CPhase.hpp:196-203                    */
+  QSV_GATE_CRX   = 16,  /* CRotationX                                            */
+  QSV_GATE_CRY   = 17,  /* CRotationY                                            */
+  QSV_GATE_CRZ   = 18,  /* CRotationZ                                            */
+  QSV_GATE_CU    = 19,  /* QControlledGate2: controlled 2x2 from mat[]           */
+  QSV_GATE_SWAP  = 20,  /* SWAP            SWAP.hpp:68-73                        */
+  QSV_GATE_ISWAP = 21,  /* iSWAP           iSWAP.hpp:68-74                       */
+  QSV_GATE_RXX   = 22,  /* RotationXX      RotationXX.hpp:136-143                */
+  QSV_GATE_RYY   = 23,  /* RotationYY      RotationYY.hpp:136-143                */
+  QSV_GATE_RZZ   = 24,  /* RotationZZ      RotationZZ.hpp:136-143                */
+  QSV_GATE_U2    = 25,  /* QGate2: dense 4x4 from mat[]                          */
+  QSV_GATE_COUNT = 26
+};
+
+/*
+ * One gate application.  The same record drives the GPU engine, the CPU
+ * oracle and the reference driver, so a circuit is a plain array of these.
+ *
+ *   q[0]      : 1-qubit gates: the qubit.  Controlled gates: the control.
+ *               2-qubit gates: the first qubit.
+ *   q[1]      : controlled gates: the target.  2-qubit gates: second qubit.
+ *   ctrl_state: control state 0 or 1 (QControlledGate2.hpp:29-36)
+ *   op        : QSV_OP_N / QSV_OP_T / QSV_OP_C
+ *   param     : index of this gate's angle in a parameter vector, or -1
+ *   has_cs    : 1 -> use (cs[0], cs[1]) = (cos, sin) verbatim, as the
+ *               (cos,sin) constructors do; 0 -> derive them from theta with
+ *               the host libm, as QRotation(theta)/QAngle(theta) do
+ *   theta     : the rotation / phase parameter
+ *   mat       : U1/CU: 2x2, U2: 4x4, column-major complex (re,im)
+ */
+typedef struct qsv_gate {
+  int32_t kind;
+  int32_t q[2];
+  int32_t ctrl_state;
+  int32_t op;
+  int32_t param;
+  int32_t has_cs;
+  int32_t reserved;
+  double  theta;
+  double  cs[2];
+  double  mat[32];
+} qsv_gate;
+
+/* ---- library ------------------------------------------------------------ */
+const char* qsv_last_error(void);
+/* Version string; also names the GPU architecture the code objects target. */
+const char* qsv_version(void);
+/* Number of visible GPUs (0 when none); never fails. */
+int qsv_device_count(void);
+
+/* ---- device-resident state (the simulate_device(T*) analogue) ------------
+ * `psi` is a device pointer to 2^n amplitudes of `dtype`.  These calls never
+ * allocate; they enqueue kernels on `stream`.                               */
+
+/* psi := |index> */
+int qsv_dev_set_basis(void* psi, int n, int dtype, uint64_t index, void* stream);
+
+/* Apply one gate with the plain per-gate kernel of its sweep class
+ * (one launch; the QObject::apply_device analogue).                         */
+int qsv_dev_apply_gate(void* psi, int n, int dtype, const qsv_gate* gate,
+                       void* stream);
+
+/* Apply `ngates` gates in order (op 'N' per gate record), one plain launch
+ * per gate (QCircuit::simulate_device, include/qclab/QCircuit.hpp:170-172). */
+int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates,
+                        size_t ngates, void* stream);
+
+/* ---- host-vector entry points (the std::vector<T>& API) -------------------
+ * `host` holds 2^n amplitudes; it is uploaded once, every gate runs on the
+ * GPU, and the result is downloaded once.  The device buffer is cached by
+ * the library between calls (qsv_release_cache frees it).                  */
+int qsv_host_apply_gates(void* host, int n, int dtype, const qsv_gate* gates,
+                         size_t ngates);
+int qsv_release_cache(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSV_H */

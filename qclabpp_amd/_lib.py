@@ -1,0 +1,97 @@
+"""ctypes binding of the C ABI in include/qsv.h (libqclabpp_amd.so).
+
+The product path has no CPU fallback: if the HIP library is not built, or no
+GPU is visible, calls raise instead of computing something else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libqclabpp_amd.so"
+HEADER = PKG.parent / "include" / "qsv.h"
+
+QSV_OK, QSV_EINVAL, QSV_EQUBIT, QSV_ENOMEM, QSV_EHIP, QSV_EDTYPE, QSV_ENODEV, QSV_EUNSUP = range(8)
+QSV_C64, QSV_C128 = 1, 2
+
+
+class QsvGate(C.Structure):
+    """Mirror of `qsv_gate` (include/qsv.h)."""
+
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("q", C.c_int32 * 2),
+        ("ctrl_state", C.c_int32),
+        ("op", C.c_int32),
+        ("param", C.c_int32),
+        ("has_cs", C.c_int32),
+        ("reserved", C.c_int32),
+        ("theta", C.c_double),
+        ("cs", C.c_double * 2),
+        ("mat", C.c_double * 32),
+    ]
+
+
+class QsvError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"qsv error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the engine; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP engine first "
+                "(python -m qclabpp_amd.build)")
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != QSV_OK:
+        msg = lib().qsv_last_error().decode(errors="replace")
+        raise QsvError(status, msg)
+
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+_I = C.c_int
+_U64 = C.c_uint64
+_D = C.c_double
+_GP = C.POINTER(QsvGate)
+_DP = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every function declared in include/qsv.h.
+SIGNATURES = {
+    "qsv_last_error": (C.c_char_p, []),
+    "qsv_version": (C.c_char_p, []),
+    "qsv_device_count": (_I, []),
+    "qsv_dev_set_basis": (_I, [_P, _I, _I, _U64, _P]),
+    "qsv_dev_apply_gate": (_I, [_P, _I, _I, _GP, _P]),
+    "qsv_dev_apply_gates": (_I, [_P, _I, _I, _GP, _SZ, _P]),
+    "qsv_host_apply_gates": (_I, [_P, _I, _I, _GP, _SZ]),
+    "qsv_release_cache": (_I, []),
+}
+
+
+def _declare(l: C.CDLL) -> None:
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(l, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def header_functions() -> list[str]:
+    """Names of the functions include/qsv.h declares."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qsv_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,111 @@
+// C ABI of the engine (declared in include/qsv.h).
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "plain.hpp"
+
+namespace qsv {
+namespace {
+
+int check_state_args(const void* psi, int n, int dtype) {
+  if (!psi) return fail(QSV_EINVAL, "null state pointer");
+  if (dtype != QSV_C64 && dtype != QSV_C128)
+    return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
+  if (n < 1 || n > 40) return fail(QSV_EINVAL, "number of qubits must be in [1, 40]");
+  return QSV_OK;
+}
+
+int elem_bytes(int dtype) { return dtype == QSV_C128 ? 16 : 8; }
+
+int apply_lowered(const KOp& op, void* psi, int n, int dtype, hipStream_t st) {
+  PlainTuning tu;
+  return dtype == QSV_C128 ? launch_plain<double>(op, psi, n, st, tu)
+                           : launch_plain<float>(op, psi, n, st, tu);
+}
+
+// Device buffer reused by the host-vector entry points.
+struct Cache {
+  std::mutex mu;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+} g_cache;
+
+}  // namespace
+}  // namespace qsv
+
+using namespace qsv;
+
+extern "C" {
+
+const char* qsv_last_error(void) { return get_error(); }
+
+const char* qsv_version(void) { return "qclabpp_amd 0.1 (HIP, gfx950)"; }
+
+int qsv_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int qsv_dev_set_basis(void* psi, int n, int dtype, uint64_t index, void* stream) {
+  QSV_TRY(check_state_args(psi, n, dtype));
+  if (index >= (uint64_t(1) << n)) return fail(QSV_EINVAL, "basis index out of range");
+  auto st = static_cast<hipStream_t>(stream);
+  return dtype == QSV_C128 ? launch_set_basis<double>(psi, n, index, st)
+                           : launch_set_basis<float>(psi, n, index, st);
+}
+
+int qsv_dev_apply_gate(void* psi, int n, int dtype, const qsv_gate* gate, void* stream) {
+  QSV_TRY(check_state_args(psi, n, dtype));
+  if (!gate) return fail(QSV_EINVAL, "null gate");
+  KOp op;
+  QSV_TRY(lower_gate(*gate, n, op));
+  return apply_lowered(op, psi, n, dtype, static_cast<hipStream_t>(stream));
+}
+
+int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates, size_t ngates,
+                        void* stream) {
+  QSV_TRY(check_state_args(psi, n, dtype));
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  std::vector<KOp> ops(ngates);
+  for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, ops[i]));
+  for (const KOp& op : ops)
+    QSV_TRY(apply_lowered(op, psi, n, dtype, static_cast<hipStream_t>(stream)));
+  return QSV_OK;
+}
+
+int qsv_host_apply_gates(void* host, int n, int dtype, const qsv_gate* gates, size_t ngates) {
+  QSV_TRY(check_state_args(host, n, dtype));
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  std::vector<KOp> ops(ngates);
+  for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, ops[i]));
+  const size_t bytes = (size_t(1) << n) * elem_bytes(dtype);
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  if (g_cache.bytes < bytes) {
+    if (g_cache.ptr) QSV_CHECK_HIP(hipFree(g_cache.ptr));
+    g_cache.ptr = nullptr;
+    g_cache.bytes = 0;
+    if (hipMalloc(&g_cache.ptr, bytes) != hipSuccess) {
+      g_cache.ptr = nullptr;
+      (void)hipGetLastError();
+      return fail(QSV_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    g_cache.bytes = bytes;
+  }
+  QSV_CHECK_HIP(hipMemcpy(g_cache.ptr, host, bytes, hipMemcpyHostToDevice));
+  for (const KOp& op : ops) QSV_TRY(apply_lowered(op, g_cache.ptr, n, dtype, nullptr));
+  QSV_CHECK_HIP(hipMemcpy(host, g_cache.ptr, bytes, hipMemcpyDeviceToHost));
+  return QSV_OK;
+}
+
+int qsv_release_cache(void) {
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  if (g_cache.ptr) QSV_CHECK_HIP(hipFree(g_cache.ptr));
+  g_cache.ptr = nullptr;
+  g_cache.bytes = 0;
+  return QSV_OK;
+}
+
+}  // extern "C"

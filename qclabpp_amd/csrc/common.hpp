@@ -1,0 +1,113 @@
+// Shared host/device definitions of the MI355X state-vector engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "qsv.h"
+
+namespace qsv {
+
+using cd = std::complex<double>;
+
+// ---- error reporting -------------------------------------------------------
+// Each C entry point returns a status; the message of the last failure is kept
+// per host thread and handed out by qsv_last_error().
+void set_error(const std::string& msg);
+const char* get_error();
+
+struct Status {
+  int code;
+};
+
+#define QSV_CHECK_HIP(expr)                                                   \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::qsv::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));   \
+      return QSV_EHIP;                                                        \
+    }                                                                         \
+  } while (0)
+
+#define QSV_TRY(expr)                                                         \
+  do {                                                                        \
+    int s_ = (expr);                                                          \
+    if (s_ != QSV_OK) return s_;                                              \
+  } while (0)
+
+inline int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+// ---- device complex helpers -------------------------------------------------
+template <typename R> struct CT;
+template <> struct CT<double> { using type = double2; };
+template <> struct CT<float>  { using type = float2; };
+
+template <typename C>
+__host__ __device__ __forceinline__ C mk(decltype(C::x) re, decltype(C::x) im) {
+  C r;
+  r.x = re;
+  r.y = im;
+  return r;
+}
+
+// a * b
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// a * b + c
+template <typename C>
+__device__ __forceinline__ C cmad(C a, C b, C c) {
+  return mk<C>(c.x + a.x * b.x - a.y * b.y, c.y + a.x * b.y + a.y * b.x);
+}
+
+// ---- lowered kernel operations ---------------------------------------------
+// Every gate kind maps to one of these sweep classes (the classes of
+// src/qgates/apply.hpp:258-371), with its Op already folded into the
+// coefficients on the host.
+enum KKind : int {
+  K_NOP = 0,
+  K_DENSE1,   // 2x2 on every pair                      (apply2)
+  K_DIAG1,    // diag(d0,d1) on every amplitude         (apply2 with RotationZ)
+  K_PHASE1,   // scale the |1> half                     (apply2b)
+  K_CDENSE1,  // 2x2 on the control==cs half            (apply4, controlled)
+  K_CPHASE,   // scale the |cs,1> quarter               (apply4b)
+  K_SWAP,     // exchange |01>,|10>, scale both         (apply4bc)
+  K_DENSE2,   // 4x4 on every quad                      (apply4, dense)
+  K_DIAG2,    // diag(d00,d01,d10,d11) on every amplitude (diagonal 4x4)
+  K_DENSEK,   // fused dense 2^k x 2^k block, k <= 5
+  K_DIAGK,    // merged diagonal over k qubits (table of 2^k phases)
+};
+
+// Row-major coefficient storage: m[r*dim + c] multiplies input c into output r.
+struct KOp {
+  int kind = K_NOP;
+  int nq = 0;          // number of qubits acted on
+  int q[8] = {0};      // qubits (MSB numbering).  Controlled: q[0]=control,
+                       // q[1]=target.  Dense2/diag2: q[0] < q[1].
+                       // Fused: ascending.
+  int cs = 1;          // control state
+  cd m[16];            // DENSE1: 4, DIAG1: 2, PHASE1/CPHASE/SWAP: m[0],
+                       // CDENSE1: 4, DENSE2: 16, DIAG2: 4
+  int blk = -1;        // K_DENSEK / K_DIAGK: index of the coefficient block
+  // adjoint support: Pauli generator of a rotation, -1 if none
+  int param = -1;
+  int gen = -1;        // gate kind of the generator owner (QSV_GATE_*)
+};
+
+// Lower one gate record to a kernel op; validates qubits against n.
+int lower_gate(const qsv_gate& g, int n, KOp& out);
+
+// Number of bytes one plain pass of `op` must read and write
+// (the algorithmic traffic of its sweep class).
+double algorithmic_bytes(const KOp& op, int n, int elem_bytes);
+
+}  // namespace qsv

@@ -1,0 +1,387 @@
+// Plain per-gate kernels: one HBM sweep per gate, one kernel per sweep class
+// of src/qgates/apply.hpp:258-371.  Index math follows the reference's masks
+// (apply.hpp:226-256): a compact loop index k gets zero bits inserted at the
+// gate's bit positions; bit(q) = n-1-q (qubit 0 is the most significant).
+//
+// Layout: the state is one contiguous array of 2^n complex values
+// (double2 for complex<double>, float2 for complex<float>) in HBM.
+// Work split: TPB threads per block, NP items per thread, items of one
+// thread TPB apart so every load instruction of a wavefront reads
+// consecutive amplitudes (a contiguous 1 KiB run for complex<double>).
+// Gates whose pair stride is below the wavefront width use the "lanes"
+// variant: each lane loads one amplitude and exchanges it with its partner
+// lane through a cross-lane shuffle, so loads stay fully contiguous.
+#include "plain.hpp"
+
+namespace qsv {
+
+namespace {
+
+__device__ __forceinline__ uint64_t ins0(uint64_t k, int b) {
+  const uint64_t lo = k & ((uint64_t(1) << b) - 1);
+  return ((k ^ lo) << 1) | lo;
+}
+
+template <typename C>
+__device__ __forceinline__ C shfl_xor_c(C v, int mask) {
+  C r;
+  r.x = __shfl_xor(v.x, mask);
+  r.y = __shfl_xor(v.y, mask);
+  return r;
+}
+
+template <typename C>
+struct M2 {
+  C a00, a01, a10, a11;
+};
+template <typename C>
+struct M4 {
+  C a[16];
+};
+
+// ---- apply2: 2x2 on pairs (a, a|s) -----------------------------------------
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_dense1(C* __restrict__ psi, uint64_t cnt, int b,
+                                                M2<C> m) {
+  const uint64_t s = uint64_t(1) << b;
+  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP], y[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t a = ins0(k, b);
+      x[j] = psi[a];
+      y[j] = psi[a | s];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t a = ins0(k, b);
+      psi[a] = cmad(m.a01, y[j], cmul(m.a00, x[j]));
+      psi[a | s] = cmad(m.a11, y[j], cmul(m.a10, x[j]));
+    }
+  }
+}
+
+// Same operation for b < 6: partner amplitude lives in lane (lane ^ 2^b).
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_dense1_lanes(C* __restrict__ psi, uint64_t nel, int b,
+                                                      M2<C> m) {
+  const int s = 1 << b;
+  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    x[j] = e < nel ? psi[e] : mk<C>(0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    const C y = shfl_xor_c(x[j], s);
+    const bool hi = (e >> b) & 1;
+    const C cs = hi ? m.a11 : m.a00;  // coefficient on own amplitude
+    const C co = hi ? m.a10 : m.a01;  // coefficient on partner amplitude
+    if (e < nel) psi[e] = cmad(co, y, cmul(cs, x[j]));
+  }
+}
+
+// ---- RotationZ-class diagonal on every amplitude ---------------------------
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_diag1(C* __restrict__ psi, uint64_t nel, int b, C d0,
+                                               C d1) {
+  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    if (e < nel) x[j] = psi[e];
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    if (e < nel) psi[e] = cmul(((e >> b) & 1) ? d1 : d0, x[j]);
+  }
+}
+
+// ---- apply2b: scale the |1> half ------------------------------------------------
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cnt, int b, C lam) {
+  const uint64_t s = uint64_t(1) << b;
+  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) x[j] = psi[ins0(k, b) | s];
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) psi[ins0(k, b) | s] = cmul(lam, x[j]);
+  }
+}
+
+// ---- apply4 (controlled): 2x2 on the control==cs half ------------------------
+// blo < bhi are the two bit positions; a1 carries the control bit, s the target.
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_cdense1(C* __restrict__ psi, uint64_t cnt, int blo,
+                                                 int bhi, uint64_t a1, uint64_t s, M2<C> m) {
+  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP], y[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t a = ins0(ins0(k, blo), bhi) | a1;
+      x[j] = psi[a];
+      y[j] = psi[a | s];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t a = ins0(ins0(k, blo), bhi) | a1;
+      psi[a] = cmad(m.a01, y[j], cmul(m.a00, x[j]));
+      psi[a | s] = cmad(m.a11, y[j], cmul(m.a10, x[j]));
+    }
+  }
+}
+
+// Controlled 2x2 with target bit < 6: shuffle across lanes, store only where
+// the control bit matches.
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_cdense1_lanes(C* __restrict__ psi, uint64_t nel, int bt,
+                                                       int bc, int cs, M2<C> m) {
+  const int s = 1 << bt;
+  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    x[j] = e < nel ? psi[e] : mk<C>(0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    const C y = shfl_xor_c(x[j], s);
+    const bool hi = (e >> bt) & 1;
+    const C cself = hi ? m.a11 : m.a00;
+    const C cother = hi ? m.a10 : m.a01;
+    if (e < nel && int((e >> bc) & 1) == cs) psi[e] = cmad(cother, y, cmul(cself, x[j]));
+  }
+}
+
+// ---- apply4b: scale one quarter ------------------------------------------------
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_cphase(C* __restrict__ psi, uint64_t cnt, int blo,
+                                                int bhi, uint64_t b1, C lam) {
+  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) x[j] = psi[ins0(ins0(k, blo), bhi) | b1];
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) psi[ins0(ins0(k, blo), bhi) | b1] = cmul(lam, x[j]);
+  }
+}
+
+// ---- apply4bc: exchange |01>,|10> and scale both by lam ---------------------
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_swap(C* __restrict__ psi, uint64_t cnt, int blo, int bhi,
+                                              C lam, int scale) {
+  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  const uint64_t sl = uint64_t(1) << blo, sh = uint64_t(1) << bhi;
+  C x[NP], y[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t i = ins0(ins0(k, blo), bhi);
+      x[j] = psi[i | sh];
+      y[j] = psi[i | sl];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t i = ins0(ins0(k, blo), bhi);
+      psi[i | sh] = scale ? cmul(lam, y[j]) : y[j];
+      psi[i | sl] = scale ? cmul(lam, x[j]) : x[j];
+    }
+  }
+}
+
+// ---- apply4 (dense): 4x4 on quads ----------------------------------------------
+// b0 = bit of the lower-numbered qubit (more significant), b1 of the other;
+// quad position p = 2*bit(b0) + bit(b1).
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_dense2(C* __restrict__ psi, uint64_t cnt, int b0, int b1,
+                                                M4<C> m) {
+  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  const uint64_t s0 = uint64_t(1) << b0, s1 = uint64_t(1) << b1;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t i = ins0(ins0(k, b1), b0);
+      const uint64_t ix[4] = {i, i | s1, i | s0, i | s0 | s1};
+      C x[4], y[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) x[p] = psi[ix[p]];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        C acc = cmul(m.a[4 * r], x[0]);
+#pragma unroll
+        for (int c = 1; c < 4; ++c) acc = cmad(m.a[4 * r + c], x[c], acc);
+        y[r] = acc;
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p) psi[ix[p]] = y[p];
+    }
+  }
+}
+
+// ---- diagonal 4x4 on every amplitude ------------------------------------------
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_diag2(C* __restrict__ psi, uint64_t nel, int b0, int b1,
+                                               C d0, C d1, C d2, C d3) {
+  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    if (e < nel) x[j] = psi[e];
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t e = e0 + uint64_t(j) * TPB;
+    if (e < nel) {
+      const int p = int(((e >> b0) & 1) << 1) | int((e >> b1) & 1);
+      const C d = p == 0 ? d0 : p == 1 ? d1 : p == 2 ? d2 : d3;
+      psi[e] = cmul(d, x[j]);
+    }
+  }
+}
+
+// ---- psi := |idx> --------------------------------------------------------------
+template <typename C>
+__global__ __launch_bounds__(TPB) void k_set_basis(C* __restrict__ psi, uint64_t nel, uint64_t idx) {
+  const uint64_t e = uint64_t(blockIdx.x) * TPB + threadIdx.x;
+  if (e < nel) psi[e] = mk<C>(e == idx ? 1 : 0, 0);
+}
+
+template <typename C>
+C to_c(const cd& z) {
+  return mk<C>(static_cast<decltype(C::x)>(z.real()), static_cast<decltype(C::x)>(z.imag()));
+}
+
+inline unsigned blocks_for(uint64_t items, int per_block) {
+  return unsigned((items + per_block - 1) / per_block);
+}
+
+}  // namespace
+
+template <typename R>
+int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTuning& tu) {
+  using C = typename CT<R>::type;
+  C* psi = static_cast<C*>(p);
+  const uint64_t N = uint64_t(1) << n;
+  constexpr int NP = 4;
+  const int per = TPB * NP;
+  auto bit = [n](int q) { return n - 1 - q; };
+  switch (op.kind) {
+    case K_NOP:
+      return QSV_OK;
+    case K_DENSE1: {
+      const M2<C> m{to_c<C>(op.m[0]), to_c<C>(op.m[1]), to_c<C>(op.m[2]), to_c<C>(op.m[3])};
+      const int b = bit(op.q[0]);
+      if (b < tu.lane_bits && n >= 6) {
+        hipLaunchKernelGGL((k_dense1_lanes<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st,
+                           psi, N, b, m);
+      } else {
+        hipLaunchKernelGGL((k_dense1<C, NP>), dim3(blocks_for(N / 2, per)), dim3(TPB), 0, st, psi,
+                           N / 2, b, m);
+      }
+      break;
+    }
+    case K_DIAG1:
+      hipLaunchKernelGGL((k_diag1<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st, psi, N,
+                         bit(op.q[0]), to_c<C>(op.m[0]), to_c<C>(op.m[1]));
+      break;
+    case K_PHASE1:
+      hipLaunchKernelGGL((k_phase1<C, NP>), dim3(blocks_for(N / 2, per)), dim3(TPB), 0, st, psi,
+                         N / 2, bit(op.q[0]), to_c<C>(op.m[0]));
+      break;
+    case K_CDENSE1: {
+      const M2<C> m{to_c<C>(op.m[0]), to_c<C>(op.m[1]), to_c<C>(op.m[2]), to_c<C>(op.m[3])};
+      const int bc = bit(op.q[0]), bt = bit(op.q[1]);
+      if (bt < tu.lane_bits && n >= 6) {
+        hipLaunchKernelGGL((k_cdense1_lanes<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st,
+                           psi, N, bt, bc, op.cs, m);
+      } else {
+        const uint64_t a1 = uint64_t(op.cs) << bc;
+        hipLaunchKernelGGL((k_cdense1<C, NP>), dim3(blocks_for(N / 4, per)), dim3(TPB), 0, st,
+                           psi, N / 4, std::min(bc, bt), std::max(bc, bt), a1,
+                           uint64_t(1) << bt, m);
+      }
+      break;
+    }
+    case K_CPHASE: {
+      const int bc = bit(op.q[0]), bt = bit(op.q[1]);
+      const uint64_t b1 = (uint64_t(op.cs) << bc) | (uint64_t(1) << bt);
+      hipLaunchKernelGGL((k_cphase<C, NP>), dim3(blocks_for(N / 4, per)), dim3(TPB), 0, st, psi,
+                         N / 4, std::min(bc, bt), std::max(bc, bt), b1, to_c<C>(op.m[0]));
+      break;
+    }
+    case K_SWAP: {
+      const int b0 = bit(op.q[0]), b1 = bit(op.q[1]);
+      const int scale = op.m[0] != cd(1.0, 0.0);
+      hipLaunchKernelGGL((k_swap<C, NP>), dim3(blocks_for(N / 4, per)), dim3(TPB), 0, st, psi,
+                         N / 4, std::min(b0, b1), std::max(b0, b1), to_c<C>(op.m[0]), scale);
+      break;
+    }
+    case K_DENSE2: {
+      M4<C> m;
+      for (int i = 0; i < 16; ++i) m.a[i] = to_c<C>(op.m[i]);
+      hipLaunchKernelGGL((k_dense2<C, 1>), dim3(blocks_for(N / 4, TPB)), dim3(TPB), 0, st, psi,
+                         N / 4, bit(op.q[0]), bit(op.q[1]), m);
+      break;
+    }
+    case K_DIAG2:
+      hipLaunchKernelGGL((k_diag2<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st, psi, N,
+                         bit(op.q[0]), bit(op.q[1]), to_c<C>(op.m[0]), to_c<C>(op.m[1]),
+                         to_c<C>(op.m[2]), to_c<C>(op.m[3]));
+      break;
+    default:
+      return fail(QSV_EUNSUP, "kernel kind " + std::to_string(op.kind) + " has no plain kernel");
+  }
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+template <typename R>
+int launch_set_basis(void* p, int n, uint64_t idx, hipStream_t st) {
+  using C = typename CT<R>::type;
+  const uint64_t N = uint64_t(1) << n;
+  hipLaunchKernelGGL((k_set_basis<C>), dim3(blocks_for(N, TPB)), dim3(TPB), 0, st,
+                     static_cast<C*>(p), N, idx);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+template int launch_plain<double>(const KOp&, void*, int, hipStream_t, const PlainTuning&);
+template int launch_plain<float>(const KOp&, void*, int, hipStream_t, const PlainTuning&);
+template int launch_set_basis<double>(void*, int, uint64_t, hipStream_t);
+template int launch_set_basis<float>(void*, int, uint64_t, hipStream_t);
+
+}  // namespace qsv
