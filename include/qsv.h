@@ -20,8 +20,12 @@
  *                                    (include/qclab/dense/transpose.hpp:12-35)
  *   - generic matrices are column-major complex (re,im interleaved),
  *     element (i,j) at [2*(i + j*size)]    (include/qclab/dense/SquareMatrix.hpp:125-132)
- *   - 2-qubit basis order is 2*bit(q0)+bit(q1) for q0 < q1
- *                                    (src/qgates/apply.hpp:289-308)
+ *   - a QGate2 matrix (QSV_GATE_U2) is applied as QGate2::apply does: for
+ *     qubits q0 < q1, matrix position p addresses the amplitude with
+ *     bit(q0) = p & 1 and bit(q1) = p >> 1 (src/qgates/apply.hpp:289-308).
+ *     For a matrix that is not symmetric under exchanging the two qubits
+ *     this differs from the kron order of QGate2::matrix() and
+ *     QCircuit::matrix(); the built-in 2-qubit gates are all symmetric.
  *
  * Errors: every function returns an int status (QSV_OK == 0); no exception
  * crosses this boundary.  qsv_last_error() returns a thread-local message
@@ -144,6 +148,39 @@ int qsv_dev_apply_gate(void* psi, int n, int dtype, const qsv_gate* gate,
  * per gate (QCircuit::simulate_device, include/qclab/QCircuit.hpp:170-172). */
 int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates,
                         size_t ngates, void* stream);
+
+/* ---- compiled circuits ----------------------------------------------------
+ * A program is a gate list lowered once to kernel operations, optionally
+ * fused: with fuse_k in 1..5 runs of gates collapse into dense 2^k x 2^k
+ * blocks (applied with MFMA for k >= 3) and runs of diagonal gates into
+ * phase tables over up to diag_k qubits (diag_k <= 10; 0 picks 8).
+ * fuse_k == 0 keeps one plain launch per gate.  Coefficients live on the
+ * device for the program's lifetime; running it allocates nothing.        */
+typedef struct qsv_program qsv_program;
+
+typedef struct qsv_program_info {
+  int64_t ngates;      /* gates given                                   */
+  int64_t nlaunch;     /* kernel launches per run                       */
+  int64_t nplain;      /* launches that are single (unfused) gates      */
+  int64_t ndiag;       /* merged diagonal passes (k >= 3)               */
+  int64_t ndense[6];   /* dense fused blocks by size k (k >= 3)         */
+  double  alg_bytes;   /* algorithmic HBM bytes of one run              */
+  double  flops;       /* useful flops of the dense fused blocks        */
+} qsv_program_info;
+
+int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,
+                       int diag_k, qsv_program** out);
+int qsv_program_run(const qsv_program* prog, void* psi, void* stream);
+int qsv_program_info_get(const qsv_program* prog, qsv_program_info* out);
+/* Launch i of one run (0-based, launch order): the qubits it acts on
+ * (ascending, k of them), and its operator as interleaved complex doubles --
+ * the 2^k x 2^k matrix row-major, or only its diagonal when *diag is set.
+ * Row bit p of the matrix is qubit qubits[k-1-p] (MSB convention of
+ * QCircuit::matrix(), include/qclab/QCircuit.hpp:121-127).  `coeffs` needs
+ * 2*4^k doubles.  Returns QSV_EINVAL past the last launch.                 */
+int qsv_program_op(const qsv_program* prog, int i, int* k, int* qubits, int* diag,
+                   double* coeffs);
+int qsv_program_destroy(qsv_program* prog);
 
 /* ---- host-vector entry points (the std::vector<T>& API) -------------------
  * `host` holds 2^n amplitudes; it is uploaded once, every gate runs on the

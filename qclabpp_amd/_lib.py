@@ -34,6 +34,20 @@ class QsvGate(C.Structure):
     ]
 
 
+class QsvProgramInfo(C.Structure):
+    """Mirror of `qsv_program_info` (include/qsv.h)."""
+
+    _fields_ = [
+        ("ngates", C.c_int64),
+        ("nlaunch", C.c_int64),
+        ("nplain", C.c_int64),
+        ("ndiag", C.c_int64),
+        ("ndense", C.c_int64 * 6),
+        ("alg_bytes", C.c_double),
+        ("flops", C.c_double),
+    ]
+
+
 class QsvError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"qsv error {code}: {msg}")
@@ -79,6 +93,11 @@ SIGNATURES = {
     "qsv_dev_apply_gate": (_I, [_P, _I, _I, _GP, _P]),
     "qsv_dev_apply_gates": (_I, [_P, _I, _I, _GP, _SZ, _P]),
     "qsv_host_apply_gates": (_I, [_P, _I, _I, _GP, _SZ]),
+    "qsv_program_create": (_I, [_I, _I, _GP, _SZ, _I, _I, C.POINTER(_P)]),
+    "qsv_program_run": (_I, [_P, _P, _P]),
+    "qsv_program_info_get": (_I, [_P, C.POINTER(QsvProgramInfo)]),
+    "qsv_program_op": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP]),
+    "qsv_program_destroy": (_I, [_P]),
     "qsv_release_cache": (_I, []),
 }
 

@@ -1,10 +1,18 @@
 // C ABI of the engine (declared in include/qsv.h).
+#include <algorithm>
 #include <cstring>
+#include <new>
 #include <mutex>
 #include <vector>
 
 #include "common.hpp"
 #include "plain.hpp"
+#include "fusion.hpp"
+#include "program.hpp"
+
+struct qsv_program {
+  qsv::Program p;
+};
 
 namespace qsv {
 namespace {
@@ -73,6 +81,87 @@ int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates, size
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, ops[i]));
   for (const KOp& op : ops)
     QSV_TRY(apply_lowered(op, psi, n, dtype, static_cast<hipStream_t>(stream)));
+  return QSV_OK;
+}
+
+int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,
+                       int diag_k, qsv_program** out) {
+  if (!out) return fail(QSV_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
+  if (n < 1 || n > 40) return fail(QSV_EINVAL, "number of qubits must be in [1, 40]");
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  if (fuse_k < 0 || fuse_k > kMaxFuse) return fail(QSV_EINVAL, "fuse_k must be in [0, 5]");
+  if (diag_k < 0 || diag_k > kMaxDiag) return fail(QSV_EINVAL, "diag_k must be in [0, 10]");
+  auto* prog = new (std::nothrow) qsv_program;
+  if (!prog) return fail(QSV_ENOMEM, "host allocation failed");
+  const int st = prog->p.build(n, dtype, gates, ngates, fuse_k, diag_k);
+  if (st != QSV_OK) {
+    delete prog;
+    return st;
+  }
+  *out = prog;
+  return QSV_OK;
+}
+
+int qsv_program_run(const qsv_program* prog, void* psi, void* stream) {
+  if (!prog) return fail(QSV_EINVAL, "null program");
+  QSV_TRY(check_state_args(psi, prog->p.n, prog->p.dtype));
+  return prog->p.run(psi, static_cast<hipStream_t>(stream));
+}
+
+int qsv_program_info_get(const qsv_program* prog, qsv_program_info* out) {
+  if (!prog || !out) return fail(QSV_EINVAL, "null argument");
+  prog->p.info(out);
+  return QSV_OK;
+}
+
+int qsv_program_op(const qsv_program* prog, int i, int* k, int* qubits, int* diag,
+                   double* coeffs) {
+  if (!prog) return fail(QSV_EINVAL, "null program");
+  if (i < 0 || size_t(i) >= prog->p.ops.size()) return fail(QSV_EINVAL, "launch index out of range");
+  const KOp& op = prog->p.ops[size_t(i)];
+  std::vector<int> q(op.q, op.q + op.nq);
+  std::sort(q.begin(), q.end());
+  const int kk = int(q.size());
+  if (k) *k = kk;
+  if (qubits)
+    for (int j = 0; j < kk; ++j) qubits[j] = q[j];
+  const bool dg = is_diagonal(op);
+  if (diag) *diag = dg ? 1 : 0;
+  if (!coeffs) return QSV_OK;
+  if (op.kind == K_DENSEK || op.kind == K_DIAGK) {
+    const Block& b = prog->p.blocks[op.blk];
+    for (size_t j = 0; j < b.m.size(); ++j) {
+      coeffs[2 * j] = b.m[j].real();
+      coeffs[2 * j + 1] = b.m[j].imag();
+    }
+    return QSV_OK;
+  }
+  KOp local = op;  // renumber to the op's own qubits
+  for (int j = 0; j < op.nq; ++j)
+    local.q[j] = int(std::lower_bound(q.begin(), q.end(), op.q[j]) - q.begin());
+  const int D = 1 << kk;
+  std::vector<cd> col(D);
+  for (int c = 0; c < D; ++c) {
+    std::fill(col.begin(), col.end(), cd(0.0, 0.0));
+    col[c] = 1.0;
+    apply_op_host(local, kk, col.data());
+    if (dg) {
+      coeffs[2 * c] = col[c].real();
+      coeffs[2 * c + 1] = col[c].imag();
+    } else {
+      for (int r = 0; r < D; ++r) {
+        coeffs[2 * (r * D + c)] = col[r].real();
+        coeffs[2 * (r * D + c) + 1] = col[r].imag();
+      }
+    }
+  }
+  return QSV_OK;
+}
+
+int qsv_program_destroy(qsv_program* prog) {
+  delete prog;
   return QSV_OK;
 }
 

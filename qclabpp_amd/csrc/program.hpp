@@ -1,0 +1,36 @@
+// A compiled circuit: its launch sequence and device-resident coefficients.
+#pragma once
+
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "fused.hpp"
+#include "fusion.hpp"
+
+struct qsv_program_info;
+
+namespace qsv {
+
+struct Program {
+  int n = 0;
+  int dtype = QSV_C128;
+  size_t ngates_in = 0;
+  std::vector<KOp> ops;
+  std::vector<Block> blocks;
+  // Block coefficients: host image built at compile time, copied to the
+  // device by upload() (first run).
+  std::vector<unsigned char> image;
+  mutable std::vector<DeviceBlock> dblocks;
+  mutable void* dbuf = nullptr;
+  mutable bool uploaded = false;
+  mutable std::mutex mu;
+
+  int upload() const;
+  int build(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k, int diag_k);
+  int run(void* psi, hipStream_t st) const;
+  void info(qsv_program_info* out) const;
+  ~Program();
+};
+
+}  // namespace qsv

@@ -95,3 +95,52 @@ def simulate_host(vec: np.ndarray, gates) -> np.ndarray:
     arr, cnt = records(gates)
     check(lib().qsv_host_apply_gates(out.ctypes.data_as(C.c_void_p), n, dt, arr, cnt))
     return out
+
+
+class Program:
+    """A compiled circuit (qsv_program): plain (fuse_k=0) or fused (1..5)."""
+
+    def __init__(self, n: int, gates, fuse_k: int = 5, dtype: str = "c128", diag_k: int = 0):
+        self.n = n
+        self.dtype = QSV_C128 if dtype in ("c128", "complex128") else QSV_C64
+        self.fuse_k = fuse_k
+        arr, cnt = records(gates)
+        h = C.c_void_p()
+        check(lib().qsv_program_create(n, self.dtype, arr, cnt, fuse_k, diag_k, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().qsv_program_destroy(h)
+            self._h = None
+
+    def run(self, psi, stream=None) -> None:
+        n, dt = _check_state(psi)
+        if n != self.n or dt != self.dtype:
+            raise ValueError("state does not match the program's qubit count / dtype")
+        check(lib().qsv_program_run(self._h, C.c_void_p(psi.data_ptr()), stream_handle(stream)))
+
+    def info(self) -> dict:
+        inf = _lib.QsvProgramInfo()
+        check(lib().qsv_program_info_get(self._h, C.byref(inf)))
+        return {"ngates": inf.ngates, "nlaunch": inf.nlaunch, "nplain": inf.nplain,
+                "ndiag": inf.ndiag, "ndense": list(inf.ndense), "alg_bytes": inf.alg_bytes,
+                "flops": inf.flops}
+
+    def launches(self):
+        """Every launch of one run, in order: (qubits, diag, coefficients) with
+        coefficients the row-major matrix or the diagonal (row bit p <->
+        qubits[k-1-p])."""
+        out = []
+        for i in range(self.info()["nlaunch"]):
+            k = C.c_int()
+            qs = (C.c_int * 16)()
+            dg = C.c_int()
+            co = (C.c_double * (2 * 4 ** 10))()
+            check(lib().qsv_program_op(self._h, i, C.byref(k), qs, C.byref(dg), co))
+            kk = k.value
+            D = 1 << kk
+            vals = np.array(co[: 2 * (D if dg.value else D * D)]).view(np.complex128)
+            out.append((list(qs[:kk]), bool(dg.value), vals if dg.value else vals.reshape(D, D)))
+        return out
