@@ -1,7 +1,10 @@
-"""Benchmark: 30-qubit random brickwork (depth 20, complex128) on one MI355X.
+"""Benchmark: 30-qubit random brickwork, depth 20, complex128, one MI355X.
 
-A step is one full simulation of the 890-gate circuit on the resident
-2^30-amplitude state (16 GiB).  Prints one JSON line.
+A step is one full simulation of the 890-gate circuit on the HBM-resident
+2^30-amplitude state (16 GiB), with the fused program (dense blocks of up to
+5 qubits on MFMA, merged diagonals).  The plain per-gate program (one sweep
+per gate) is timed on the same state for the fused-vs-plain ratio.  Prints
+one JSON line.
 """
 from __future__ import annotations
 
@@ -15,90 +18,149 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+PEAK_HBM = 8000.0      # GB/s, MI355X HBM3E
+PEAK_FP64 = 78.6       # TFLOP/s dense FP64 matrix
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--depth", type=int, default=20)
+    ap.add_argument("--fuse-k", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-gates", type=int, default=45,
+                    help="gates of the circuit prefix the CPU baseline simulates")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     return ap.parse_args()
+
+
+def timed(torch, fn, reps, stream):
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps  # ms
+
+
+def cpu_baseline(n, circ, count):
+    """QCLAB++'s own QCircuit::simulate (oracle/_ref, built from the
+    reference sources) on the first `count` gates, from |0...0>; the CPU
+    restatement when that library is absent."""
+    import oracle
+    from qclabpp_amd import gates as G
+    sample = circ[:count]
+    if oracle.ref_available():
+        secs = oracle.ref_time_simulate_zero(n, sample)
+        kind, cores = "reference", oracle.ref_threads()
+    else:
+        import numpy as np
+        psi = np.zeros(1 << n, np.complex128)
+        psi[0] = 1
+        t0 = time.perf_counter()
+        oracle.apply(n, sample, psi)
+        secs = time.perf_counter() - t0
+        kind, cores = "port", os.cpu_count()
+    return {"value": len(sample) / secs, "unit": "gate-passes/s", "cores": cores, "kind": kind,
+            "sample": f"first {len(sample)} gates of the n={n} brickwork (its first layer), "
+                      f"QCircuit::simulate on |0...0>, {secs:.2f} s"}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the engine is single-GPU; sharding is a design note only).", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
         return 2
-    import numpy as np
     import torch
 
     from qclabpp_amd import engine as E
+    from qclabpp_amd import gates as G
     from qclabpp_amd import workloads as W
-    from qclabpp_amd._lib import lib
 
     n, depth = args.n, args.depth
     circ = W.brickwork(n, depth)
-    nfull = sum(1 for g in circ if g.nb_qubits == 1)
-    nhalf = len(circ) - nfull
     S = (1 << n) * 16
-    alg_bytes = nfull * 2 * S + nhalf * S
+    stream = torch.cuda.current_stream()
+
+    t0 = time.perf_counter()
+    fused = E.Program(n, circ, fuse_k=args.fuse_k)
+    plain = E.Program(n, circ, fuse_k=0)
+    compile_s = time.perf_counter() - t0
+    fi, pi_ = fused.info(), plain.info()
 
     psi = E.new_state(n, torch.complex128)
-    stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
-        E.apply_gates(psi, circ)
+        fused.run(psi)
+        plain.run(psi)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.steps):
-        E.apply_gates(psi, circ)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1) / args.steps
-    gates_per_s = len(circ) / (ms / 1e3)
 
-    # Dominant kernel: dense 1-qubit sweep (RX/RY); time launches on a
-    # high-stride qubit with HIP events on the launch stream.
-    from qclabpp_amd import gates as G
-    probe = [G.RotationX(3, 0.1)] * 10
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    E.apply_gates(psi, probe[:2])
-    e0.record(stream)
-    E.apply_gates(psi, probe)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    k_ms = e0.elapsed_time(e1) / len(probe)
-    k_bytes = 2 * S
-    achieved = k_bytes / (k_ms / 1e3) / 1e9
+    # ---- timed region: K fused steps ------------------------------------------
+    t_wall = time.perf_counter()
+    ms_fused = timed(torch, lambda: fused.run(psi), args.steps, stream)
+    wall = time.perf_counter() - t_wall
+    ms_plain = timed(torch, lambda: plain.run(psi), args.plain_steps, stream)
+
+    # ---- dominant kernels, HIP events on the launch stream ---------------------
+    # fused: one 5-qubit dense block (MFMA), repeated
+    blk_gates = [G.RotationX(q, 0.1 * q) for q in range(12, 17)] + \
+                [G.CX(q, q + 1) for q in range(12, 16)]
+    blk = E.Program(n, blk_gates, fuse_k=5)
+    assert blk.info()["ndense"][5] == 1 and blk.info()["nlaunch"] == 1
+    blk.run(psi)
+    ms_blk = timed(torch, lambda: blk.run(psi), 10, stream)
+    # plain: the dense 1-qubit sweep (RX/RY of the brickwork) on a high-stride qubit
+    one = E.Program(n, [G.RotationY(3, 0.2)], fuse_k=0)
+    one.run(psi)
+    ms_one = timed(torch, lambda: one.run(psi), 10, stream)
+
+    gbps_blk = 2 * S / (ms_blk / 1e3) / 1e9
+    tflops_blk = 8.0 * 32 * (1 << n) / (ms_blk / 1e3) / 1e12
+    gbps_one = 2 * S / (ms_one / 1e3) / 1e9
 
     out = {
-        "metric": "30q brickwork gate-passes/s (complex128)",
-        "value": gates_per_s,
+        "metric": "30q brickwork gate-passes/s, fused (k<=5, MFMA) complex128",
+        "value": len(circ) / (ms_fused / 1e3),
         "unit": "gate-passes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": ms_fused,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128",
-        "data": "synthetic (seeded brickwork, |0...0>)",
-        "config": {"workload": f"brickwork n={n} depth={depth} plain per-gate", "gates": len(circ)},
-        "hbm_GBps_plain": alg_bytes / (ms / 1e3) / 1e9,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None},
+        "data": "synthetic: seeded brickwork circuit (SplitMix64 seed 20261202), state |0...0>",
+        "config": {"workload": f"brickwork n={n} depth={depth} ({len(circ)} gates), complex128, "
+                               f"fused k<={args.fuse_k}", "gates": len(circ)},
+        "fused": {"launches": fi["nlaunch"], "dense_blocks": fi["ndense"],
+                  "alg_bytes": fi["alg_bytes"], "hbm_GBps": fi["alg_bytes"] / (ms_fused / 1e3) / 1e9,
+                  "mfma_TFLOPs": fi["flops"] / (ms_fused / 1e3) / 1e12},
+        "plain": {"launches": pi_["nlaunch"], "ms_per_step": ms_plain,
+                  "gate_passes_per_s": len(circ) / (ms_plain / 1e3),
+                  "hbm_GBps": pi_["alg_bytes"] / (ms_plain / 1e3) / 1e9},
+        "fused_speedup_vs_plain": ms_plain / ms_fused,
+        "kernels": {
+            "fused_dense_k5_mfma": {"ms": ms_blk, "GBps": gbps_blk, "TFLOPs": tflops_blk,
+                                    "frac_hbm": gbps_blk / PEAK_HBM, "frac_fp64": tflops_blk / PEAK_FP64},
+            "plain_dense1": {"ms": ms_one, "GBps": gbps_one, "frac_hbm": gbps_one / PEAK_HBM},
+        },
+        "roofline": {"bound": "hbm", "achieved": gbps_blk, "peak": PEAK_HBM, "unit": "GB/s",
+                     "frac": gbps_blk / PEAK_HBM, "traffic": None},
         "cpu_baseline": None,
+        "compile_s": compile_s,
         "wall_s": wall,
     }
+    if not args.no_cpu:
+        del psi
+        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(n, circ, args.cpu_gates)
     print(json.dumps(out))
     return 0
 

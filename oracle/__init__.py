@@ -73,7 +73,24 @@ def ref_lib() -> C.CDLL:
         _r.ref_circuit_apply.argtypes = [C.c_int, C.c_int, gp, C.c_size_t, C.c_int, C.c_void_p,
                                          C.POINTER(C.c_double)]
         _r.ref_circuit_matrix.argtypes = [C.c_int, C.c_int, gp, C.c_size_t, C.c_void_p]
+        _r.ref_time_simulate_zero.argtypes = [C.c_int, C.c_int, gp, C.c_size_t,
+                                              C.POINTER(C.c_double)]
     return _r
+
+
+def ref_time_simulate_zero(n: int, gates, dtype=np.complex128) -> float:
+    """Wall seconds of the reference's QCircuit::simulate on |0...0>."""
+    arr, cnt = records(gates)
+    secs = C.c_double(0)
+    st = ref_lib().ref_time_simulate_zero(n, _dtype_code(np.zeros(1, dtype)), arr, cnt,
+                                          C.byref(secs))
+    if st:
+        raise RuntimeError(f"reference driver status {st}")
+    return secs.value
+
+
+def ref_threads() -> int:
+    return int(ref_lib().ref_max_threads())
 
 
 def _dtype_code(arr: np.ndarray) -> int:

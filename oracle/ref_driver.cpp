@@ -244,6 +244,27 @@ int ref_circuit_matrix(int n, int dtype, const qsv_gate* g, size_t ng, void* out
   return QSV_EDTYPE;
 }
 
+// QCircuit::simulate on |0...0> built inside the library (no copies), as
+// test/timings/run.hpp:49-56 does; `secs` gets the simulate() wall time.
+int ref_time_simulate_zero(int n, int dtype, const qsv_gate* g, size_t ng, double* secs) {
+  if ((!g && ng) || !secs || n < 1 || n > 34) return QSV_EINVAL;
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    qclab::QCircuit<T> circ(n);
+    const int st = build_circuit(n, g, ng, circ);
+    if (st) return st;
+    std::vector<T> v(size_t(1) << n, T(0));
+    v[0] = 1;
+    const auto t0 = std::chrono::steady_clock::now();
+    circ.simulate(v);
+    *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return QSV_OK;
+  };
+  if (dtype == QSV_C128) return run(std::complex<double>());
+  if (dtype == QSV_C64) return run(std::complex<float>());
+  return QSV_EDTYPE;
+}
+
 int ref_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
