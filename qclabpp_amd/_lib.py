@@ -65,6 +65,15 @@ def lib() -> C.CDLL:
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build the HIP engine first "
                 "(python -m qclabpp_amd.build)")
+        # torch ships its own libamdhip64 (soname libamdhip64.so.7, loaded
+        # under the file name libamdhip64.so).  Loaded first, it satisfies the
+        # engine's libamdhip64.so.7 dependency, so the process has one HIP
+        # runtime; loading the engine first would map /opt/rocm's runtime as
+        # well, and the second runtime then finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(str(LIB_PATH))
         _declare(_lib)
     return _lib

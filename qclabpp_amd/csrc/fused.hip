@@ -13,7 +13,8 @@
 // LDS and stores the tile with the same coalesced pattern.
 //
 // MFMA operand layouts (16x16x4, 64 lanes): A[i][k] in lane i + 16k,
-// B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i/4), element i%4.
+// B[k][j] in lane j + 16k; D[i][j] in lane j + 16*(i/4), element i%4 for
+// FP32, and in lane j + 16*(i%4), element i/4 for FP64 (MF<R>::row).
 //
 // Diagonal block (k <= 10 qubits): each amplitude is multiplied by the table
 // entry its block bits select; the table sits in LDS.
@@ -27,17 +28,22 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename R> struct MF;
+// row(lane, r): the output row (0..15) that element r of `lane` holds; the
+// column is lane % 16 for both.  Measured on gfx950: the FP64 variant
+// interleaves rows (lane/16 + 4r), the FP32 variant groups them (4*(lane/16) + r).
 template <> struct MF<double> {
   using acc = v4d;
   static __device__ __forceinline__ acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct MF<float> {
   using acc = v4f;
   static __device__ __forceinline__ acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 __device__ __forceinline__ uint32_t pdep32(uint32_t x, uint32_t mask) {
@@ -126,12 +132,14 @@ __global__ __launch_bounds__(256) void k_fused_mfma(typename CT<R>::type* __rest
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) acc[mt] = MF<R>::mma(areg[mt][s], b[s], acc[mt]);
     __syncthreads();
+    R* tre = reinterpret_cast<R*>(&tile[wave][0]);
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const int a0 = 8 * mt + 2 * (lane >> 4);
-      tile[wave][lA[a0] + lg] = mk<C>(acc[mt][0], acc[mt][1]);
-      tile[wave][lA[a0 + 1] + lg] = mk<C>(acc[mt][2], acc[mt][3]);
-    }
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * mt + MF<R>::row(lane, r);  // real row: 2*amp + (re|im)
+        tre[2 * (lA[row >> 1] + lg) + (row & 1)] = acc[mt][r];
+      }
     __syncthreads();
     if (act) {
 #pragma unroll
