@@ -34,7 +34,41 @@ def parse():
     ap.add_argument("--cpu-gates", type=int, default=45,
                     help="gates of the circuit prefix the CPU baseline simulates")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the dominant kernels and a 16 GiB copy (for rocprofv3 --pmc)")
     return ap.parse_args()
+
+
+def dominant_programs(E, G, n):
+    """One 5-qubit dense block (the fused path's MFMA kernel) and one dense
+    1-qubit sweep (the plain path's RX/RY kernel), each a single launch."""
+    blk_gates = [G.RotationX(q, 0.1 * q) for q in range(12, 17)] + \
+                [G.CX(q, q + 1) for q in range(12, 16)]
+    blk = E.Program(n, blk_gates, fuse_k=5)
+    assert blk.info()["ndense"][5] == 1 and blk.info()["nlaunch"] == 1
+    one = E.Program(n, [G.RotationY(3, 0.2)], fuse_k=0)
+    return blk, one
+
+
+def kernels_only(n):
+    """Launch pattern for counter collection: 5 x k=5 block, 5 x dense
+    1-qubit sweep, 3 x copy of 2^n complex128 (2^n * 16 bytes read and written)."""
+    import torch
+
+    from qclabpp_amd import engine as E
+    from qclabpp_amd import gates as G
+    blk, one = dominant_programs(E, G, n)
+    psi = E.new_state(n, torch.complex128)
+    for _ in range(5):
+        blk.run(psi)
+    for _ in range(5):
+        one.run(psi)
+    dst = torch.empty_like(psi)
+    for _ in range(3):
+        dst.copy_(psi)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_only": True, "n": n, "state_bytes": (1 << n) * 16}))
+    return 0
 
 
 def timed(torch, fn, reps, stream):
@@ -78,6 +112,8 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
         return 2
+    if args.kernels_only:
+        return kernels_only(args.n)
     import torch
 
     from qclabpp_amd import engine as E
@@ -108,15 +144,9 @@ def main():
     ms_plain = timed(torch, lambda: plain.run(psi), args.plain_steps, stream)
 
     # ---- dominant kernels, HIP events on the launch stream ---------------------
-    # fused: one 5-qubit dense block (MFMA), repeated
-    blk_gates = [G.RotationX(q, 0.1 * q) for q in range(12, 17)] + \
-                [G.CX(q, q + 1) for q in range(12, 16)]
-    blk = E.Program(n, blk_gates, fuse_k=5)
-    assert blk.info()["ndense"][5] == 1 and blk.info()["nlaunch"] == 1
+    blk, one = dominant_programs(E, G, n)
     blk.run(psi)
     ms_blk = timed(torch, lambda: blk.run(psi), 10, stream)
-    # plain: the dense 1-qubit sweep (RX/RY of the brickwork) on a high-stride qubit
-    one = E.Program(n, [G.RotationY(3, 0.2)], fuse_k=0)
     one.run(psi)
     ms_one = timed(torch, lambda: one.run(psi), 10, stream)
 
