@@ -71,6 +71,19 @@ def kernels_only(n):
     return 0
 
 
+def measured_traffic(n):
+    """HBM bytes per launch of the k=5 block kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by a copy kernel), or
+    None when that summary does not cover this size."""
+    f = ROOT / "profiles" / "fused_k5_mfma_n30_c128_hbm_traffic.md"
+    if n != 30 or not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split(":")[1])
+    return None
+
+
 def timed(torch, fn, reps, stream):
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -182,7 +195,8 @@ def main():
             "plain_dense1": {"ms": ms_one, "GBps": gbps_one, "frac_hbm": gbps_one / PEAK_HBM},
         },
         "roofline": {"bound": "hbm", "achieved": gbps_blk, "peak": PEAK_HBM, "unit": "GB/s",
-                     "frac": gbps_blk / PEAK_HBM, "traffic": None},
+                     "frac": gbps_blk / PEAK_HBM, "traffic": measured_traffic(n),
+                     "kernel": "k_fused_mfma<double,5>", "alg_bytes_per_launch": 2 * S},
         "cpu_baseline": None,
         "compile_s": compile_s,
         "wall_s": wall,
