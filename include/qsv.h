@@ -125,6 +125,22 @@ typedef struct qsv_gate {
   double  mat[32];
 } qsv_gate;
 
+/*
+ * One term coeff * P of a Pauli-sum observable, P a tensor product of
+ * nfactors single-qubit Paulis ('X', 'Y', 'Z'; 'I' factors are ignored) on
+ * distinct qubits.  QCLAB++ has no observables; Paulis follow its PauliX /
+ * PauliY / PauliZ matrices (PauliX.hpp:46-49, PauliY.hpp:46-49,
+ * PauliZ.hpp:46-49) and its MSB qubit order.
+ */
+#define QSV_MAX_PAULI_FACTORS 16
+typedef struct qsv_pauli_term {
+  double  coeff;
+  int32_t nfactors;
+  int32_t reserved;
+  int32_t qubits[QSV_MAX_PAULI_FACTORS];
+  char    paulis[QSV_MAX_PAULI_FACTORS];
+} qsv_pauli_term;
+
 /* ---- library ------------------------------------------------------------ */
 const char* qsv_last_error(void);
 /* Version string; also names the GPU architecture the code objects target. */
@@ -181,6 +197,34 @@ int qsv_program_info_get(const qsv_program* prog, qsv_program_info* out);
 int qsv_program_op(const qsv_program* prog, int i, int* k, int* qubits, int* diag,
                    double* coeffs);
 int qsv_program_destroy(qsv_program* prog);
+
+/* ---- observables and gradients (no counterpart in the reference) ---------
+ * All reductions accumulate in FP64 (also for QSV_C64 states) on a fixed
+ * grid, so results are reproducible bit for bit.  These calls synchronise
+ * `stream` before returning their host results.                           */
+
+/* values[t] = <psi|P_t|psi> (coefficient not applied; may be NULL);
+ * *total = sum_t coeff_t values[t] (may be NULL).                          */
+int qsv_dev_expectation(const void* psi, int n, int dtype, const qsv_pauli_term* terms,
+                        size_t nterms, double* values, double* total, void* stream);
+/* <a|b> = sum conj(a_x) b_x. */
+int qsv_dev_inner(const void* a, const void* b, int n, int dtype, double* re, double* im,
+                  void* stream);
+/* out = sum_t coeff_t P_t in  (out must not alias in; at most 1024 terms). */
+int qsv_dev_apply_pauli_sum(const void* in, void* out, int n, int dtype,
+                            const qsv_pauli_term* terms, size_t nterms, void* stream);
+/* Adjoint gradient of E = <psi|H|psi>, psi = U(theta) psi0.
+ *   psi    : in: psi0; out: psi0 recomputed by the reverse sweep
+ *   lam    : scratch state of the same size
+ *   theta  : nparams angles overriding each parametrised gate's theta
+ *            (gate.param indexes theta), or NULL to use the records' own
+ *   fuse_k : fusion width of the forward pass (0 = plain)
+ *   energy : E;  grad[p] = dE/dtheta_p summed over the gates sharing p
+ * Differentiable kinds: RX RY RZ RXX RYY RZZ CRX CRY CRZ P CP, Op 'N'.    */
+int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_gate* gates,
+                             size_t ngates, const double* theta, size_t nparams,
+                             const qsv_pauli_term* terms, size_t nterms, int fuse_k,
+                             double* energy, double* grad, void* stream);
 
 /* ---- host-vector entry points (the std::vector<T>& API) -------------------
  * `host` holds 2^n amplitudes; it is uploaded once, every gate runs on the

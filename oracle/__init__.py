@@ -48,16 +48,14 @@ def oracle_lib() -> C.CDLL:
 
 
 def _declare_extra(o) -> None:
+    from qclabpp_amd._lib import QsvPauliTerm
     gp = C.POINTER(QsvGate)
     dp = C.POINTER(C.c_double)
-    for name, args in {
-        "oracle_expect_tfim": [C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_double, dp],
-        "oracle_block_matrix": [C.c_int, C.POINTER(C.c_int), gp, C.c_size_t, C.c_void_p],
-        "oracle_param_shift": [C.c_int, C.c_int, gp, C.c_size_t, C.c_double, C.c_double,
-                               C.POINTER(C.c_int), C.c_size_t, dp, dp],
-    }.items():
-        if hasattr(o, name):
-            getattr(o, name).argtypes = args
+    tp = C.POINTER(QsvPauliTerm)
+    o.oracle_expectation.argtypes = [C.c_int, C.c_int, C.c_void_p, tp, C.c_size_t, dp]
+    o.oracle_energy.argtypes = [C.c_int, gp, C.c_size_t, dp, tp, C.c_size_t, dp]
+    o.oracle_param_shift.argtypes = [C.c_int, gp, C.c_size_t, dp, C.c_size_t, tp, C.c_size_t,
+                                     C.POINTER(C.c_int), C.c_size_t, dp]
 
 
 def ref_available() -> bool:
@@ -155,3 +153,43 @@ def ref_matrix(n: int, gates, dtype=np.complex128) -> np.ndarray:
     if st:
         raise RuntimeError(f"reference driver status {st}")
     return out.reshape(N, N, order="F")
+
+
+def expectation(n: int, psi: np.ndarray, H) -> np.ndarray:
+    """Per-term <psi|P_t|psi> through the restated PauliX/Y/Z gates."""
+    psi = np.ascontiguousarray(psi)
+    arr, cnt = H.records()
+    vals = (C.c_double * max(1, cnt))()
+    st = oracle_lib().oracle_expectation(n, _dtype_code(psi), _ptr(psi), arr, cnt, vals)
+    if st:
+        raise RuntimeError(f"oracle status {st}")
+    return np.array(vals[:cnt])
+
+
+def _theta_arg(theta):
+    th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
+    return th, th.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def energy(n: int, gates, H, theta) -> float:
+    arr, cnt = records(gates)
+    harr, hcnt = H.records()
+    th, thp = _theta_arg(theta)
+    e = C.c_double()
+    st = oracle_lib().oracle_energy(n, arr, cnt, thp, harr, hcnt, C.byref(e))
+    if st:
+        raise RuntimeError(f"oracle status {st}")
+    return e.value
+
+
+def param_shift(n: int, gates, H, theta, which) -> np.ndarray:
+    """CPU parameter-shift dE/dtheta_p for p in `which`."""
+    arr, cnt = records(gates)
+    harr, hcnt = H.records()
+    th, thp = _theta_arg(theta)
+    w = (C.c_int * max(1, len(which)))(*which)
+    g = (C.c_double * max(1, len(which)))()
+    st = oracle_lib().oracle_param_shift(n, arr, cnt, thp, th.size, harr, hcnt, w, len(which), g)
+    if st:
+        raise RuntimeError(f"oracle status {st}")
+    return np.array(g[:len(which)])

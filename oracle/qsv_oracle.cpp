@@ -23,6 +23,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -437,6 +438,99 @@ int oracle_apply_gates_op(int n, int dtype, const qsv_gate* g, size_t ng, int op
   }
   for (auto& x : seq) x.op = op;
   return oracle_apply_gates(n, dtype, seq.data(), seq.size(), state);
+}
+
+// ---- observables (no reference counterpart; built from its gate semantics) --
+// <psi|P|psi> for each Pauli string: apply PauliX/Y/Z (lambda_PauliX/Y/Z via
+// apply2/apply2b, restated above) to a copy and take the inner product with
+// psi, accumulated in double.
+int oracle_expectation(int n, int dtype, const void* state, const qsv_pauli_term* terms,
+                       size_t nterms, double* values) {
+  if (!state || (!terms && nterms) || !values) return QSV_EINVAL;
+  const size_t N = size_t(1) << n;
+  auto run = [&](auto* psi) -> int {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(psi)>>;
+    std::vector<T> tmp(N);
+    for (size_t t = 0; t < nterms; ++t) {
+      std::memcpy(tmp.data(), psi, N * sizeof(T));
+      for (int f = 0; f < terms[t].nfactors; ++f) {
+        qsv_gate g{};
+        g.op = QSV_OP_N;
+        g.q[0] = terms[t].qubits[f];
+        const char p = terms[t].paulis[f];
+        if (p == 'I') continue;
+        g.kind = p == 'X' ? QSV_GATE_X : p == 'Y' ? QSV_GATE_Y : p == 'Z' ? QSV_GATE_Z : -1;
+        if (g.kind < 0) return QSV_EINVAL;
+        const int st = apply_gate(n, g, tmp.data());
+        if (st) return st;
+      }
+      double re = 0;
+#pragma omp parallel for reduction(+ : re) schedule(static)
+      for (int64_t x = 0; x < int64_t(N); ++x) {
+        const std::complex<double> a(psi[x]), b(tmp[x]);
+        re += (std::conj(a) * b).real();
+      }
+      values[t] = re;
+    }
+    return QSV_OK;
+  };
+  if (dtype == QSV_C128) return run(static_cast<const std::complex<double>*>(state));
+  if (dtype == QSV_C64) return run(static_cast<const std::complex<float>*>(state));
+  return QSV_EDTYPE;
+}
+
+// E(theta) = <psi|H|psi>, psi = U(theta)|0...0>, parameters overriding the
+// gates' angles (gate.param indexes theta).
+static int energy_at(int n, const qsv_gate* g, size_t ng, const double* theta,
+                     const qsv_pauli_term* terms, size_t nterms, double* e) {
+  std::vector<qsv_gate> gs(g, g + ng);
+  for (auto& x : gs)
+    if (x.param >= 0) {
+      x.theta = theta[x.param];
+      x.has_cs = 0;
+    }
+  std::vector<std::complex<double>> psi(size_t(1) << n);
+  psi[0] = 1;
+  int st = apply_gates(n, gs.data(), gs.size(), psi.data());
+  if (st) return st;
+  std::vector<double> v(nterms);
+  st = oracle_expectation(n, QSV_C128, psi.data(), terms, nterms, v.data());
+  if (st) return st;
+  double s = 0;
+  for (size_t t = 0; t < nterms; ++t) s += terms[t].coeff * v[t];
+  *e = s;
+  return QSV_OK;
+}
+
+int oracle_energy(int n, const qsv_gate* g, size_t ng, const double* theta,
+                  const qsv_pauli_term* terms, size_t nterms, double* e) {
+  if (!e || (!g && ng)) return QSV_EINVAL;
+  return energy_at(n, g, ng, theta, terms, nterms, e);
+}
+
+// Parameter-shift gradient (exact for gates exp(-i theta P/2), P^2 = I, and
+// for Phase): dE/dtheta_p = [E(theta + pi/2 e_p) - E(theta - pi/2 e_p)] / 2,
+// for the parameters listed in `which`.
+int oracle_param_shift(int n, const qsv_gate* g, size_t ng, const double* theta, size_t nparams,
+                       const qsv_pauli_term* terms, size_t nterms, const int* which,
+                       size_t nwhich, double* grad) {
+  if (!theta || !grad || (!g && ng) || (!which && nwhich)) return QSV_EINVAL;
+  std::vector<double> th(theta, theta + nparams);
+  const double h = 3.14159265358979323846 / 2;
+  for (size_t i = 0; i < nwhich; ++i) {
+    const int p = which[i];
+    if (p < 0 || size_t(p) >= nparams) return QSV_EINVAL;
+    double ep = 0, em = 0;
+    th[p] = theta[p] + h;
+    int st = energy_at(n, g, ng, th.data(), terms, nterms, &ep);
+    if (st) return st;
+    th[p] = theta[p] - h;
+    st = energy_at(n, g, ng, th.data(), terms, nterms, &em);
+    if (st) return st;
+    th[p] = theta[p];
+    grad[i] = (ep - em) / 2;
+  }
+  return QSV_OK;
 }
 
 }  // extern "C"

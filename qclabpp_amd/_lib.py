@@ -34,6 +34,18 @@ class QsvGate(C.Structure):
     ]
 
 
+class QsvPauliTerm(C.Structure):
+    """Mirror of `qsv_pauli_term` (include/qsv.h)."""
+
+    _fields_ = [
+        ("coeff", C.c_double),
+        ("nfactors", C.c_int32),
+        ("reserved", C.c_int32),
+        ("qubits", C.c_int32 * 16),
+        ("paulis", C.c_byte * 16),
+    ]
+
+
 class QsvProgramInfo(C.Structure):
     """Mirror of `qsv_program_info` (include/qsv.h)."""
 
@@ -108,6 +120,11 @@ SIGNATURES = {
     "qsv_program_op": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP]),
     "qsv_program_destroy": (_I, [_P]),
     "qsv_release_cache": (_I, []),
+    "qsv_dev_expectation": (_I, [_P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _DP, _DP, _P]),
+    "qsv_dev_inner": (_I, [_P, _P, _I, _I, _DP, _DP, _P]),
+    "qsv_dev_apply_pauli_sum": (_I, [_P, _P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _P]),
+    "qsv_dev_adjoint_gradient": (_I, [_P, _P, _I, _I, _GP, _SZ, _DP, _SZ, C.POINTER(QsvPauliTerm),
+                                      _SZ, _I, _DP, _DP, _P]),
 }
 
 

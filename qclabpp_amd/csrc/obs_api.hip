@@ -1,0 +1,304 @@
+// C ABI: expectation values, Pauli-sum products and the adjoint gradient.
+//
+// Adjoint differentiation (Jones & Gacon, arXiv:2009.02823): with
+// phi = U_N..U_1 psi0 and lam = H phi, sweep k = N..1:
+//   dE/dtheta_k from <lam|G_k|phi>, then phi <- U_k^H phi, lam <- U_k^H lam,
+// where the un-apply U_k^H is the gate applied with Op ConjTrans -- the
+// reference's own inverse (QCircuit::apply with Op::ConjTrans reverses the
+// gate order and conjugate-transposes each gate, include/qclab/QCircuit.hpp:137-141).
+// States are recomputed by inversion instead of being stored: the sweep needs
+// two state buffers whatever the depth.
+//
+// Rotations exp(-i theta P / 2) give dE/dtheta = Im<lam|P|phi>; Phase/CPhase
+// diag(1, e^{i theta}) give dE/dtheta = -2 Im<lam|Pi_1|phi> (Pi_1 projects the
+// target, and the control, onto the phased state).
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "fusion.hpp"
+#include "observables.hpp"
+#include "plain.hpp"
+#include "program.hpp"
+
+namespace qsv {
+namespace {
+
+// Device scratch for reduction partials and term tables, reused between
+// calls (callers are serialised by the mutex).
+struct Scratch {
+  std::mutex mu;
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return QSV_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, need) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return fail(QSV_ENOMEM, "scratch allocation of " + std::to_string(need) + " bytes failed");
+    }
+    bytes = need;
+    return QSV_OK;
+  }
+} g_scratch;
+
+int check_state(const void* psi, int n, int dtype) {
+  if (!psi) return fail(QSV_EINVAL, "null state pointer");
+  if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
+  if (n < 1 || n > 40) return fail(QSV_EINVAL, "number of qubits must be in [1, 40]");
+  return QSV_OK;
+}
+
+int terms_to_ops(const qsv_pauli_term* terms, size_t nterms, int n, std::vector<PauliOp>& ops) {
+  if (nterms && !terms) return fail(QSV_EINVAL, "null term array");
+  ops.resize(nterms);
+  for (size_t i = 0; i < nterms; ++i) QSV_TRY(pauli_from_term(terms[i], n, ops[i]));
+  return QSV_OK;
+}
+
+template <typename R>
+int inner(const void* a, const void* b, int n, const PauliOp& g, void* partials, hipStream_t st) {
+  return launch_pauli_inner<R>(a, b, n, g, partials, st);
+}
+
+int inner_any(int dtype, const void* a, const void* b, int n, const PauliOp& g, void* partials,
+              hipStream_t st) {
+  return dtype == QSV_C128 ? inner<double>(a, b, n, g, partials, st)
+                           : inner<float>(a, b, n, g, partials, st);
+}
+
+int plain_any(int dtype, const KOp& op, void* psi, int n, hipStream_t st) {
+  PlainTuning tu;
+  return dtype == QSV_C128 ? launch_plain<double>(op, psi, n, st, tu)
+                           : launch_plain<float>(op, psi, n, st, tu);
+}
+
+// Generator of a differentiable gate and the weight turning z = <lam|G|phi>
+// into dE/dtheta = wr*Re(z) + wi*Im(z).
+bool generator(const qsv_gate& g, int n, PauliOp& G, double& wr, double& wi) {
+  G = PauliOp();
+  auto bit = [n](int q) { return uint64_t(1) << (n - 1 - q); };
+  const uint64_t b0 = bit(g.q[0]);
+  const uint64_t b1 = g.kind >= QSV_GATE_CX ? bit(g.q[1]) : 0;
+  wr = 0;
+  wi = 1;  // rotations: Im<lam|P|phi> (times the P's own phase below)
+  switch (g.kind) {
+    case QSV_GATE_RX: G.flip = b0; break;
+    case QSV_GATE_RY: G.flip = b0; G.zmask = b0; G.phase_re = 0; G.phase_im = 1; break;
+    case QSV_GATE_RZ: G.zmask = b0; break;
+    case QSV_GATE_RXX: G.flip = b0 | b1; break;
+    case QSV_GATE_RYY: G.flip = b0 | b1; G.zmask = b0 | b1; G.phase_re = -1; break;
+    case QSV_GATE_RZZ: G.zmask = b0 | b1; break;
+    case QSV_GATE_CRX: case QSV_GATE_CRY: case QSV_GATE_CRZ:
+      G.pmask = b0;
+      G.pval = g.ctrl_state ? b0 : 0;
+      if (g.kind == QSV_GATE_CRX) G.flip = b1;
+      if (g.kind == QSV_GATE_CRY) { G.flip = b1; G.zmask = b1; G.phase_re = 0; G.phase_im = 1; }
+      if (g.kind == QSV_GATE_CRZ) G.zmask = b1;
+      break;
+    case QSV_GATE_P:
+      G.pmask = b0; G.pval = b0; wi = -2;
+      break;
+    case QSV_GATE_CP:
+      G.pmask = b0 | b1; G.pval = (g.ctrl_state ? b0 : 0) | b1; wi = -2;
+      break;
+    default:
+      return false;
+  }
+  return true;
+}
+
+template <typename R>
+int expectation_impl(const void* psi, int n, const std::vector<PauliOp>& ops, double* values,
+                     hipStream_t st) {
+  // diagonal, projector-free terms: 32 per pass; others: one pass each
+  std::vector<int> diag, off;
+  for (int i = 0; i < int(ops.size()); ++i)
+    (ops[i].flip == 0 && ops[i].pmask == 0 ? diag : off).push_back(i);
+  const size_t rows = ops.size();
+  const size_t part_bytes = rows * kRedBlocks * sizeof(double2);
+  std::lock_guard<std::mutex> lk(g_scratch.mu);
+  QSV_TRY(g_scratch.ensure(part_bytes + rows * sizeof(double2) + 64 * sizeof(uint64_t)));
+  auto* partials = static_cast<double2*>(g_scratch.p);
+  auto* red = partials + rows * kRedBlocks;
+  void* tmp = red + rows;
+  // rows are laid out in the order: diagonal chunks, then off-diagonal terms
+  std::vector<int> row_of(ops.size());
+  size_t r = 0;
+  for (size_t c = 0; c < diag.size(); c += 32) {
+    std::vector<PauliOp> chunk;
+    for (size_t j = c; j < std::min(diag.size(), c + 32); ++j) {
+      chunk.push_back(ops[diag[j]]);
+      row_of[diag[j]] = int(r + (j - c));
+    }
+    QSV_TRY(launch_diag_expect<R>(psi, n, chunk, partials + r * kRedBlocks, tmp, st));
+    r += chunk.size();
+  }
+  for (int i : off) {
+    row_of[i] = int(r);
+    QSV_TRY(launch_pauli_inner<R>(psi, psi, n, ops[i], partials + r * kRedBlocks, st));
+    ++r;
+  }
+  QSV_TRY(launch_reduce_rows(partials, int(rows), red, st));
+  std::vector<double2> host(rows);
+  QSV_CHECK_HIP(hipMemcpyAsync(host.data(), red, rows * sizeof(double2), hipMemcpyDeviceToHost, st));
+  QSV_CHECK_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const double2 z = host[row_of[i]];
+    // <P> = phase * z (real for a Hermitian P)
+    values[i] = ops[i].phase_re * z.x - ops[i].phase_im * z.y;
+  }
+  return QSV_OK;
+}
+
+}  // namespace
+}  // namespace qsv
+
+using namespace qsv;
+
+extern "C" {
+
+int qsv_dev_expectation(const void* psi, int n, int dtype, const qsv_pauli_term* terms,
+                        size_t nterms, double* values, double* total, void* stream) {
+  QSV_TRY(check_state(psi, n, dtype));
+  std::vector<PauliOp> ops;
+  QSV_TRY(terms_to_ops(terms, nterms, n, ops));
+  std::vector<double> v(nterms);
+  auto st = static_cast<hipStream_t>(stream);
+  if (nterms)
+    QSV_TRY(dtype == QSV_C128 ? expectation_impl<double>(psi, n, ops, v.data(), st)
+                              : expectation_impl<float>(psi, n, ops, v.data(), st));
+  double s = 0;
+  for (size_t i = 0; i < nterms; ++i) {
+    if (values) values[i] = v[i];
+    s += ops[i].coeff * v[i];
+  }
+  if (total) *total = s;
+  return QSV_OK;
+}
+
+int qsv_dev_inner(const void* a, const void* b, int n, int dtype, double* re, double* im,
+                  void* stream) {
+  QSV_TRY(check_state(a, n, dtype));
+  QSV_TRY(check_state(b, n, dtype));
+  auto st = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lk(g_scratch.mu);
+  QSV_TRY(g_scratch.ensure((kRedBlocks + 1) * sizeof(double2)));
+  auto* partials = static_cast<double2*>(g_scratch.p);
+  QSV_TRY(inner_any(dtype, a, b, n, PauliOp(), partials, st));
+  QSV_TRY(launch_reduce_rows(partials, 1, partials + kRedBlocks, st));
+  double2 z;
+  QSV_CHECK_HIP(hipMemcpyAsync(&z, partials + kRedBlocks, sizeof(z), hipMemcpyDeviceToHost, st));
+  QSV_CHECK_HIP(hipStreamSynchronize(st));
+  if (re) *re = z.x;
+  if (im) *im = z.y;
+  return QSV_OK;
+}
+
+int qsv_dev_apply_pauli_sum(const void* in, void* out, int n, int dtype,
+                            const qsv_pauli_term* terms, size_t nterms, void* stream) {
+  QSV_TRY(check_state(in, n, dtype));
+  QSV_TRY(check_state(out, n, dtype));
+  if (in == out) return fail(QSV_EINVAL, "input and output must be different buffers");
+  std::vector<PauliOp> ops;
+  QSV_TRY(terms_to_ops(terms, nterms, n, ops));
+  if (ops.empty()) return fail(QSV_EINVAL, "empty Pauli sum");
+  if (ops.size() > size_t(kMaxPauliTerms)) return fail(QSV_EINVAL, "Pauli sum has too many terms");
+  std::vector<DevPauli> dt;
+  std::vector<int> gs;
+  group_terms(ops, dt, gs);
+  auto st = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lk(g_scratch.mu);
+  const size_t tb = dt.size() * sizeof(DevPauli);
+  QSV_TRY(g_scratch.ensure(tb + gs.size() * sizeof(int)));
+  void* dterms = g_scratch.p;
+  void* dgs = static_cast<unsigned char*>(g_scratch.p) + tb;
+  QSV_CHECK_HIP(hipMemcpyAsync(dterms, dt.data(), tb, hipMemcpyHostToDevice, st));
+  QSV_CHECK_HIP(hipMemcpyAsync(dgs, gs.data(), gs.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  const int ng = int(gs.size()) - 1;
+  QSV_TRY(dtype == QSV_C128
+              ? launch_pauli_apply<double>(in, out, n, dterms, dgs, ng, int(dt.size()), st)
+              : launch_pauli_apply<float>(in, out, n, dterms, dgs, ng, int(dt.size()), st));
+  QSV_CHECK_HIP(hipStreamSynchronize(st));  // host term tables go out of scope
+  return QSV_OK;
+}
+
+int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_gate* gates,
+                             size_t ngates, const double* theta, size_t nparams,
+                             const qsv_pauli_term* terms, size_t nterms, int fuse_k,
+                             double* energy, double* grad, void* stream) {
+  QSV_TRY(check_state(psi, n, dtype));
+  QSV_TRY(check_state(lam, n, dtype));
+  if (psi == lam) return fail(QSV_EINVAL, "psi and lam must be different buffers");
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  if (nparams && !grad) return fail(QSV_EINVAL, "null gradient output");
+  auto st = static_cast<hipStream_t>(stream);
+  // parameter overrides
+  std::vector<qsv_gate> gs(gates, gates + ngates);
+  for (auto& g : gs) {
+    if (g.param >= 0) {
+      if (size_t(g.param) >= nparams) return fail(QSV_EINVAL, "gate parameter index out of range");
+      if (g.op != QSV_OP_N) return fail(QSV_EUNSUP, "parametrised gates must use Op 'N'");
+      if (theta) {
+        g.theta = theta[g.param];
+        g.has_cs = 0;
+      }
+    }
+  }
+  // forward: phi = U psi0
+  Program fwd;
+  QSV_TRY(fwd.build(n, dtype, gs.data(), gs.size(), fuse_k, 0));
+  QSV_TRY(fwd.run(psi, st));
+  // lam = H phi
+  QSV_TRY(qsv_dev_apply_pauli_sum(psi, lam, n, dtype, terms, nterms, stream));
+  // per-gate generators and un-apply ops
+  std::vector<PauliOp> gens;
+  std::vector<int> gen_gate;
+  std::vector<double> wr, wi;
+  std::vector<KOp> inv(gs.size());
+  for (size_t k = 0; k < gs.size(); ++k) {
+    qsv_gate c = gs[k];
+    c.op = QSV_OP_C;
+    QSV_TRY(lower_gate(c, n, inv[k]));
+  }
+  const size_t nrows = 1 + gs.size();  // energy row + at most one per gate
+  std::lock_guard<std::mutex> lk(g_scratch.mu);
+  QSV_TRY(g_scratch.ensure(nrows * (kRedBlocks + 1) * sizeof(double2)));
+  auto* partials = static_cast<double2*>(g_scratch.p);
+  auto* red = partials + nrows * kRedBlocks;
+  // energy row: <phi|lam>
+  QSV_TRY(inner_any(dtype, psi, lam, n, PauliOp(), partials, st));
+  int rows = 1;
+  std::vector<int> row_param(1, -1);
+  std::vector<double> row_wr(1, 0), row_wi(1, 0);
+  for (size_t kk = gs.size(); kk-- > 0;) {
+    PauliOp G;
+    double a = 0, b = 0;
+    if (gs[kk].param >= 0) {
+      if (!generator(gs[kk], n, G, a, b))
+        return fail(QSV_EUNSUP, "gate kind " + std::to_string(gs[kk].kind) + " is not differentiable");
+      QSV_TRY(inner_any(dtype, lam, psi, n, G, partials + size_t(rows) * kRedBlocks, st));
+      // fold the generator's own phase into the weights: z_true = phase * z
+      row_param.push_back(gs[kk].param);
+      row_wr.push_back(a * G.phase_re + b * G.phase_im);
+      row_wi.push_back(-a * G.phase_im + b * G.phase_re);
+      ++rows;
+    }
+    QSV_TRY(plain_any(dtype, inv[kk], psi, n, st));
+    QSV_TRY(plain_any(dtype, inv[kk], lam, n, st));
+  }
+  QSV_TRY(launch_reduce_rows(partials, rows, red, st));
+  std::vector<double2> host(rows);
+  QSV_CHECK_HIP(hipMemcpyAsync(host.data(), red, rows * sizeof(double2), hipMemcpyDeviceToHost, st));
+  QSV_CHECK_HIP(hipStreamSynchronize(st));
+  if (energy) *energy = host[0].x;
+  for (size_t p = 0; p < nparams; ++p) grad[p] = 0;
+  for (int r = 1; r < rows; ++r) grad[row_param[r]] += row_wr[r] * host[r].x + row_wi[r] * host[r].y;
+  return QSV_OK;
+}
+
+}  // extern "C"
