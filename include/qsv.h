@@ -226,6 +226,62 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
                              const qsv_pauli_term* terms, size_t nterms, int fuse_k,
                              double* energy, double* grad, void* stream);
 
+/* ---- batched circuits (no counterpart in the reference) --------------------
+ * Many independent circuits of uneven qubit counts, each started from
+ * |0...0>, simulated together.  Their final states are packed in one device
+ * buffer: circuit c occupies amplitudes [offsets[c], offsets[c+1]),
+ * offsets[c+1] = offsets[c] + 2^n_c.  Gates of circuit c are
+ * gates[gate_offsets[c] .. gate_offsets[c+1]).                            */
+typedef struct qsv_batch qsv_batch;
+
+typedef struct qsv_batch_opts {
+  int32_t tile_bits;   /* LDS tile width for circuits above small_bits (0 = default) */
+  int32_t small_bits;  /* circuits with n <= small_bits run whole in LDS (0 = largest) */
+  int32_t fuse_k;      /* 0: gates as given; 1: merge runs of 1-qubit gates;
+                          2: also fuse into blocks of at most 2 qubits           */
+  int32_t reserved;
+} qsv_batch_opts;
+
+typedef struct qsv_batch_info_t {
+  int64_t  ncirc;
+  int64_t  nsmall;     /* circuits run whole in LDS                         */
+  int64_t  npasses;    /* tile passes over all circuits                     */
+  int64_t  nops;       /* tile operations over all passes                   */
+  int64_t  nlevels;    /* dependent launches of one run                     */
+  uint64_t total_amplitudes;
+  qsv_batch_opts opts; /* the options in effect                             */
+} qsv_batch_info_t;
+
+typedef struct qsv_tune_result {
+  qsv_batch_opts opts;
+  double ms;            /* mean time of one run                             */
+  double max_abs_diff;  /* vs the first candidate's states                  */
+} qsv_tune_result;
+
+int qsv_batch_create(int dtype, int ncirc, const int* nqubits, const qsv_gate* gates,
+                     const size_t* gate_offsets, const qsv_batch_opts* opts, qsv_batch** out);
+/* offsets may be NULL; otherwise receives ncirc+1 amplitude offsets.      */
+int qsv_batch_info(const qsv_batch* batch, qsv_batch_info_t* info, uint64_t* offsets);
+/* Simulate every circuit into `packed`.  use_graph != 0 replays a hipGraph
+ * captured at the first call for this buffer (capture allocates nothing in
+ * the graph: counters are reset by a kernel node).                        */
+int qsv_batch_run(qsv_batch* batch, void* packed, int use_graph, void* stream);
+/* Per-circuit completion time of the last run, seconds from its start.    */
+int qsv_batch_latencies(qsv_batch* batch, double* seconds, void* stream);
+int qsv_batch_destroy(qsv_batch* batch);
+/* Time every candidate (tile_bits in 10..13 that fit, fuse_k in 0..2) on
+ * `packed` (reps runs each, graph replay), check that all candidates give
+ * the same states (max |difference| reported per candidate), and return the
+ * fastest in *best.  `scratch` holds a copy of the first candidate's states
+ * (same size as packed).  results may be NULL.                            */
+int qsv_batch_autotune(int dtype, int ncirc, const int* nqubits, const qsv_gate* gates,
+                       const size_t* gate_offsets, void* packed, void* scratch, int reps,
+                       qsv_batch_opts* best, qsv_tune_result* results, int max_results,
+                       int* nresults, void* stream);
+/* max_i |a_i - b_i| over `count` amplitudes (FP64 result). */
+int qsv_dev_max_abs_diff(const void* a, const void* b, uint64_t count, int dtype, double* out,
+                         void* stream);
+
 /* ---- host-vector entry points (the std::vector<T>& API) -------------------
  * `host` holds 2^n amplitudes; it is uploaded once, every gate runs on the
  * GPU, and the result is downloaded once.  The device buffer is cached by

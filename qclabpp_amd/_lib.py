@@ -46,6 +46,27 @@ class QsvPauliTerm(C.Structure):
     ]
 
 
+class QsvBatchOpts(C.Structure):
+    """Mirror of `qsv_batch_opts` (include/qsv.h)."""
+
+    _fields_ = [("tile_bits", C.c_int32), ("small_bits", C.c_int32), ("fuse_k", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class QsvBatchInfo(C.Structure):
+    """Mirror of `qsv_batch_info_t` (include/qsv.h)."""
+
+    _fields_ = [("ncirc", C.c_int64), ("nsmall", C.c_int64), ("npasses", C.c_int64),
+                ("nops", C.c_int64), ("nlevels", C.c_int64), ("total_amplitudes", C.c_uint64),
+                ("opts", QsvBatchOpts)]
+
+
+class QsvTuneResult(C.Structure):
+    """Mirror of `qsv_tune_result` (include/qsv.h)."""
+
+    _fields_ = [("opts", QsvBatchOpts), ("ms", C.c_double), ("max_abs_diff", C.c_double)]
+
+
 class QsvProgramInfo(C.Structure):
     """Mirror of `qsv_program_info` (include/qsv.h)."""
 
@@ -123,6 +144,16 @@ SIGNATURES = {
     "qsv_dev_expectation": (_I, [_P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _DP, _DP, _P]),
     "qsv_dev_inner": (_I, [_P, _P, _I, _I, _DP, _DP, _P]),
     "qsv_dev_apply_pauli_sum": (_I, [_P, _P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _P]),
+    "qsv_batch_create": (_I, [_I, _I, C.POINTER(_I), _GP, C.POINTER(_SZ), C.POINTER(QsvBatchOpts),
+                              C.POINTER(_P)]),
+    "qsv_batch_info": (_I, [_P, C.POINTER(QsvBatchInfo), C.POINTER(_U64)]),
+    "qsv_batch_run": (_I, [_P, _P, _I, _P]),
+    "qsv_batch_latencies": (_I, [_P, _DP, _P]),
+    "qsv_batch_destroy": (_I, [_P]),
+    "qsv_batch_autotune": (_I, [_I, _I, C.POINTER(_I), _GP, C.POINTER(_SZ), _P, _P, _I,
+                                C.POINTER(QsvBatchOpts), C.POINTER(QsvTuneResult), _I,
+                                C.POINTER(_I), _P]),
+    "qsv_dev_max_abs_diff": (_I, [_P, _P, _U64, _I, _DP, _P]),
     "qsv_dev_adjoint_gradient": (_I, [_P, _P, _I, _I, _GP, _SZ, _DP, _SZ, C.POINTER(QsvPauliTerm),
                                       _SZ, _I, _DP, _DP, _P]),
 }

@@ -43,7 +43,7 @@ def sources() -> list[Path]:
 
 
 def _compile(src: Path) -> Path:
-    out = OBJ / (src.stem + ".o")
+    out = OBJ / (src.name + ".o")
     cmd = [_hipcc(), *HIP_FLAGS, "-x", "hip", "-c", str(src), "-o", str(out)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

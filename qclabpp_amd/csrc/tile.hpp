@@ -1,0 +1,65 @@
+// LDS tile passes: a pass loads a tile of 2^t amplitudes (the values of t
+// chosen bits, the others fixed) into LDS, applies a list of gates that act
+// only on the tile's bits (diagonal gates may act on any bit), and stores it
+// back.  Used by the batch engine: a circuit whose state fits one tile runs
+// entirely in LDS; a larger one runs as a sequence of passes, each absorbing
+// as many gates as the dependencies allow.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace qsv {
+
+constexpr int kTileMaxBits = 14;  // 2^14 complex<float> = 128 KiB of LDS
+constexpr int kTileThreads = 256;
+
+// One operation inside a tile.  Bit operands: >= 0 is a tile-local bit,
+// < 0 encodes an address bit outside the tile as ~bit (diagonal ops only).
+enum TKind : int32_t {
+  T_DENSE1 = 1,  // 2x2 on local bit a                      m[0..3]
+  T_DIAG1 = 2,   // multiply by m[bit(a)]                    m[0..1]
+  T_DIAG2 = 3,   // multiply by m[2*bit(a) + bit(b)]         m[0..3]
+  T_CDENSE1 = 4, // 2x2 on local bit a where bit(b) == cs    m[0..3]
+  T_SWAP = 5,    // exchange |a=1,b=0> and |a=0,b=1>, times m[0]
+  T_DENSE2 = 6,  // 4x4 on local bits (a, b), row = 2*bit(a)+bit(b)  m[0..15]
+};
+
+struct TOp {
+  int32_t kind;
+  int32_t a, b, cs;
+  double m[32];  // complex coefficients, interleaved
+};
+
+struct TPass {
+  int32_t circuit;
+  int32_t t;            // tile bits
+  int32_t op_begin, op_end;
+  int32_t init_zero;    // first pass of a circuit: the tile starts as |0...0>
+  int32_t last;         // last pass of its circuit
+  int32_t nruns;
+  uint8_t tbits[kTileMaxBits];
+  uint8_t rdst[kTileMaxBits + 2];
+  uint8_t rlen[kTileMaxBits + 2];
+  uint64_t ntiles;
+};
+
+struct TCircuit {
+  uint64_t offset;  // amplitude offset in the packed buffer
+  int32_t n;
+  int32_t npasses;
+};
+
+struct TItem {
+  int32_t pass;
+  int32_t pad;
+  uint64_t tile;
+};
+
+// Host compile of one circuit into passes over tiles of `t` bits.
+// `merge1q` multiplies runs of 1-qubit gates on the same qubit into one.
+int compile_tiled(const std::vector<KOp>& ops, int n, int t, bool merge1q, int circuit_id,
+                  std::vector<TPass>& passes, std::vector<TOp>& tops);
+
+}  // namespace qsv

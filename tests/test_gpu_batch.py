@@ -1,0 +1,95 @@
+"""Batched circuits (LDS tile passes, graph replay, autotuner) against the
+oracle, circuit by circuit."""
+import numpy as np
+import pytest
+
+import oracle
+from qclabpp_amd import batch as B
+from qclabpp_amd import gates as G
+from qclabpp_amd import workloads as W
+from qclabpp_amd.rng import SplitMix64
+from tests.gatecases import random_unitary
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_circuit(n, seed):
+    rs = np.random.default_rng(seed)
+    out = [G.Hadamard(q) for q in range(n)]
+    for i in range(3 * n):
+        a, b = (int(x) for x in rs.choice(n, 2, replace=False)) if n > 1 else (0, 0)
+        th = float(rs.uniform(-3, 3))
+        out += [G.RotationY(a, th), G.RotationZ(b, -th), G.Phase45(a)]
+        if n > 1:
+            out += [G.CX(a, b), G.CZ(a, b), G.CPhase(b, a, th), G.SWAP(a, b), G.iSWAP(a, b),
+                    G.CRotationX(a, b, th, controlState=0), G.RotationZZ(min(a, b), max(a, b), th),
+                    G.RotationXX(min(a, b), max(a, b), -th),
+                    G.QGate2(min(a, b), max(a, b), random_unitary(4, seed * 100 + i))]
+    return out
+
+
+def zero(n, dtype):
+    v = np.zeros(1 << n, dtype)
+    v[0] = 1
+    return v
+
+
+def check_all(batch, packed, circuits, nq, tol):
+    import torch
+    torch.cuda.synchronize()
+    host = packed.cpu().numpy()
+    worst = 0.0
+    for c, (circ, n) in enumerate(zip(circuits, nq)):
+        want = oracle.apply(n, circ, zero(n, np.complex128))
+        got = host[int(batch.offsets[c]):int(batch.offsets[c + 1])]
+        worst = max(worst, float(np.abs(got - want).max()))
+    assert worst < tol, worst
+    return worst
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-12), ("c64", 5e-5)])
+@pytest.mark.parametrize("fuse_k", [0, 1, 2])
+@pytest.mark.parametrize("tile_bits,small_bits", [(6, 6), (8, 10), (10, 13)])
+def test_mixed_batch_vs_oracle(dtype, tol, fuse_k, tile_bits, small_bits):
+    print(f"batch {dtype} fuse_k={fuse_k} tile={tile_bits} small={small_bits}", flush=True)
+    nq = [1, 2, 3, 5, 7, 9, 11, 12, 14, 15, 8, 6, 13, 4, 10]
+    circuits = [mixed_circuit(n, 17 + i) for i, n in enumerate(nq)]
+    b = B.Batch(circuits, nq, dtype=dtype, tile_bits=tile_bits, small_bits=small_bits, fuse_k=fuse_k)
+    packed = b.new_buffer()
+    b.run(packed, use_graph=False)
+    check_all(b, packed, circuits, nq, tol)
+
+
+def test_graph_replays_match_eager():
+    import torch
+    spec = W.hea_batch(48, seed=7, nmin=4, nmax=17, lmin=2, lmax=6)
+    b = B.Batch(spec.circuits, spec.nq, dtype="c64", tile_bits=10, small_bits=12, fuse_k=1)
+    eager = b.new_buffer()
+    b.run(eager, use_graph=False)
+    torch.cuda.synchronize()
+    ref = eager.clone()
+    packed = b.new_buffer()
+    for rep in range(4):
+        packed.zero_()                       # eager work between the replays
+        b.run(packed, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(packed, ref), rep
+    check_all(b, packed, spec.circuits, spec.nq, 5e-5)
+    lat = b.latencies()
+    assert np.all(lat > 0) and np.all(np.isfinite(lat))
+
+
+def test_autotuner_candidates_agree():
+    import torch
+    spec = W.hea_batch(24, seed=3, nmin=6, nmax=16, lmin=2, lmax=5)
+    nq = spec.nq
+    total = sum(1 << n for n in nq)
+    packed = torch.empty(total, dtype=torch.complex128, device="cuda")
+    scratch = torch.empty_like(packed)
+    best, results = B.autotune(spec.circuits, nq, packed, scratch, dtype="c128", reps=1)
+    assert len(results) == 12  # tile bits 10..13 (complex128 caps at 13) x fuse_k 0..2
+    assert max(r["max_abs_diff"] for r in results) < 1e-12
+    b = B.Batch(spec.circuits, nq, dtype="c128", **best)
+    out = b.new_buffer()
+    b.run(out)
+    check_all(b, out, spec.circuits, nq, 1e-12)
