@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--adjoint-layers", type=int, default=10)
     ap.add_argument("--ps-params", type=int, default=1,
                     help="parameters checked against the CPU parameter-shift gradient")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="circuits of the batched HEA sweep (0 skips it)")
+    ap.add_argument("--batch-runs", type=int, default=5)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the dominant kernels and a 16 GiB copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -120,6 +123,55 @@ def adjoint_leg(args):
         ps = oracle.param_shift(n, circ, H, theta, which)
         out["param_shift_cpu"] = {"params": which, "s": time.perf_counter() - t0,
                                   "max_abs_adjoint_minus_ps": float(np.abs(g_adj[which] - ps).max())}
+    return out
+
+
+def batch_leg(args):
+    """512 hardware-efficient-ansatz circuits (n ~ U{8..22}, L ~ U{4..16},
+    per layer RY+RZ on every qubit then a CZ ladder), complex64, from
+    |0...0> into one packed buffer: autotuned tile and fusion width, then
+    timed graph replays.  Reports circuits/s and the p50/p99 of per-circuit
+    completion times within a run."""
+    import numpy as np
+    import torch
+
+    from qclabpp_amd import batch as B
+    from qclabpp_amd import workloads as W
+
+    spec = W.hea_batch(args.batch)
+    total = sum(1 << n for n in spec.nq)
+    packed = torch.empty(total, dtype=torch.complex64, device="cuda")
+    scratch = torch.empty_like(packed)
+    t0 = time.perf_counter()
+    best, results = B.autotune(spec.circuits, spec.nq, packed, scratch, dtype="c64", reps=2)
+    tune_s = time.perf_counter() - t0
+    del scratch
+    b = B.Batch(spec.circuits, spec.nq, dtype="c64", **best)
+    b.run(packed)  # capture
+    stream = torch.cuda.current_stream()
+    ms = timed(torch, lambda: b.run(packed), args.batch_runs, stream)
+    lat = b.latencies() * 1e3
+    gates = sum(len(c) for c in spec.circuits)
+    out = {"workload": f"{args.batch} HEA circuits, n in [8,22], layers in [4,16], complex64",
+           "state_bytes": total * 8, "gates": gates, "ms_per_run": ms,
+           "circuits_per_s": args.batch / (ms / 1e3), "gates_per_s": gates / (ms / 1e3),
+           "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+           "tuned": best, "tune_s": tune_s,
+           "candidates": [{k: r[k] for k in ("tile_bits", "fuse_k", "ms")} for r in results],
+           "max_abs_diff_between_candidates": max(r["max_abs_diff"] for r in results),
+           "passes": b.info["npasses"], "levels": b.info["nlevels"], "small_circuits": b.info["nsmall"]}
+    if not args.no_cpu:
+        import oracle
+        sample = [i for i in range(args.batch)][:16]
+        t0 = time.perf_counter()
+        for i in sample:
+            n = spec.nq[i]
+            oracle.ref_time_simulate_zero(n, spec.circuits[i], np.complex64) if oracle.ref_available() \
+                else None
+        secs = time.perf_counter() - t0
+        out["cpu_reference"] = {"circuits_per_s": len(sample) / secs, "sample": f"first {len(sample)} "
+                                "circuits, QCircuit::simulate (reference, complex<float>)",
+                                "cores": oracle.ref_threads() if oracle.ref_available() else None}
     return out
 
 
@@ -257,6 +309,9 @@ def main():
     torch.cuda.empty_cache()
     if args.adjoint_n > 0:
         out["adjoint"] = adjoint_leg(args)
+        torch.cuda.empty_cache()
+    if args.batch > 0:
+        out["batch"] = batch_leg(args)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, circ, args.cpu_gates)

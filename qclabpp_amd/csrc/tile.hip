@@ -54,9 +54,12 @@ __device__ void apply_top(const TOp& o, C* L, uint32_t size, uint64_t base) {
       break;
     }
     case T_DIAG2: {
-      const C d[4] = {coef<C>(o, 0), coef<C>(o, 1), coef<C>(o, 2), coef<C>(o, 3)};
-      for (uint32_t e = tid; e < size; e += kTileThreads)
-        L[e] = cmul(d[2 * obit(o.a, e, base) + obit(o.b, e, base)], L[e]);
+      const C d0 = coef<C>(o, 0), d1 = coef<C>(o, 1), d2 = coef<C>(o, 2), d3 = coef<C>(o, 3);
+      for (uint32_t e = tid; e < size; e += kTileThreads) {
+        const int ba = obit(o.a, e, base), bb = obit(o.b, e, base);
+        const C d = ba ? (bb ? d3 : d2) : (bb ? d1 : d0);
+        L[e] = cmul(d, L[e]);
+      }
       break;
     }
     case T_CDENSE1: {
@@ -87,13 +90,15 @@ __device__ void apply_top(const TOp& o, C* L, uint32_t size, uint64_t base) {
       const int lo = o.a < o.b ? o.a : o.b, hi = o.a < o.b ? o.b : o.a;
       for (uint32_t p = tid; p < size / 4; p += kTileThreads) {
         const uint32_t i = uint32_t(ins0(ins0(p, lo), hi));
-        const uint32_t ix[4] = {i, i | (1u << o.b), i | (1u << o.a), i | (1u << o.a) | (1u << o.b)};
-        C x[4];
-        for (int q = 0; q < 4; ++q) x[q] = L[ix[q]];
+        const uint32_t i1 = i | (1u << o.b), i2 = i | (1u << o.a), i3 = i2 | (1u << o.b);
+        const C x0 = L[i], x1 = L[i1], x2 = L[i2], x3 = L[i3];
+#pragma unroll
         for (int r = 0; r < 4; ++r) {
-          C acc = cmul(coef<C>(o, 4 * r), x[0]);
-          for (int c = 1; c < 4; ++c) acc = cmad(coef<C>(o, 4 * r + c), x[c], acc);
-          L[ix[r]] = acc;
+          C acc = cmul(coef<C>(o, 4 * r), x0);
+          acc = cmad(coef<C>(o, 4 * r + 1), x1, acc);
+          acc = cmad(coef<C>(o, 4 * r + 2), x2, acc);
+          acc = cmad(coef<C>(o, 4 * r + 3), x3, acc);
+          L[r == 0 ? i : r == 1 ? i1 : r == 2 ? i2 : i3] = acc;
         }
       }
       break;
@@ -101,23 +106,6 @@ __device__ void apply_top(const TOp& o, C* L, uint32_t size, uint64_t base) {
     default:
       break;
   }
-}
-
-__device__ __forceinline__ uint64_t deposit(uint64_t w, const TPass& P) {
-  uint64_t a = 0;
-  int sh = 0;
-  for (int r = 0; r < P.nruns; ++r) {
-    a |= ((w >> sh) & ((uint64_t(1) << P.rlen[r]) - 1)) << P.rdst[r];
-    sh += P.rlen[r];
-  }
-  return a;
-}
-
-__device__ __forceinline__ uint64_t pdep_tile(uint32_t e, const TPass& P) {
-  uint64_t off = 0;
-  for (int i = 0; i < P.t; ++i)
-    if (e >> i & 1u) off |= uint64_t(1) << P.tbits[i];
-  return off;
 }
 
 template <typename C>
@@ -128,38 +116,58 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_level(
   extern __shared__ unsigned char smem[];
   C* L = reinterpret_cast<C*>(smem);
   __shared__ unsigned long long cur;
+  __shared__ TPass sp;  // the current item's pass (dynamically indexed in LDS, not in registers)
   for (;;) {
     if (threadIdx.x == 0) cur = atomicAdd(ticket, 1ull);
     __syncthreads();
     const unsigned long long it = cur;
-    __syncthreads();
     if (it >= nitems) break;
     const TItem item = items[it];
-    const TPass P = passes[item.pass];
-    C* state = psi + circs[P.circuit].offset;
-    const uint64_t base = deposit(item.tile, P);
-    const uint32_t size = 1u << P.t;
-    for (uint32_t e = threadIdx.x; e < size; e += kTileThreads) {
-      if (P.init_zero)
-        L[e] = mk<C>((base == 0 && e == 0) ? 1 : 0, 0);
-      else
-        L[e] = state[base + pdep_tile(e, P)];
+    if (threadIdx.x == 0) sp = passes[item.pass];
+    __syncthreads();
+    const int t = sp.t;
+    C* state = psi + circs[sp.circuit].offset;
+    uint64_t base = 0;
+    {
+      int sh = 0;
+      for (int r = 0; r < sp.nruns; ++r) {
+        base |= ((item.tile >> sh) & ((uint64_t(1) << sp.rlen[r]) - 1)) << sp.rdst[r];
+        sh += sp.rlen[r];
+      }
+    }
+    const uint32_t size = 1u << t;
+    // element e = threadIdx.x + 256*k: offset = pdep(low 8 bits) | pdep(k << 8)
+    uint64_t lane_off = 0;
+    for (int i = 0; i < t && i < 8; ++i)
+      if (threadIdx.x >> i & 1u) lane_off |= uint64_t(1) << sp.tbits[i];
+    const int init = sp.init_zero;
+    for (uint32_t e = threadIdx.x, k = 0; e < size; e += kTileThreads, ++k) {
+      uint64_t off = lane_off;
+      for (int i = 8; i < t; ++i)
+        if (k >> (i - 8) & 1u) off |= uint64_t(1) << sp.tbits[i];
+      L[e] = init ? mk<C>((base == 0 && e == 0) ? 1 : 0, 0) : state[base + off];
     }
     __syncthreads();
-    for (int o = P.op_begin; o < P.op_end; ++o) {
+    const int ob = sp.op_begin, oe = sp.op_end;
+    for (int o = ob; o < oe; ++o) {
       apply_top<C>(ops[o], L, size, base);
       __syncthreads();
     }
-    for (uint32_t e = threadIdx.x; e < size; e += kTileThreads) state[base + pdep_tile(e, P)] = L[e];
-    if (P.last) {
+    for (uint32_t e = threadIdx.x, k = 0; e < size; e += kTileThreads, ++k) {
+      uint64_t off = lane_off;
+      for (int i = 8; i < t; ++i)
+        if (k >> (i - 8) & 1u) off |= uint64_t(1) << sp.tbits[i];
+      state[base + off] = L[e];
+    }
+    if (sp.last) {
       __threadfence();
       __syncthreads();
       if (threadIdx.x == 0) {
-        const unsigned int prev = atomicAdd(&done_tiles[P.circuit], 1u);
-        if (uint64_t(prev) + 1 == P.ntiles) stamps[P.circuit] = wall_clock64();
+        const unsigned int prev = atomicAdd(&done_tiles[sp.circuit], 1u);
+        if (uint64_t(prev) + 1 == sp.ntiles) stamps[sp.circuit] = wall_clock64();
       }
     }
-    __syncthreads();
+    __syncthreads();  // all threads are done with `sp` and `cur` before they are rewritten
   }
 }
 
