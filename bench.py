@@ -251,14 +251,16 @@ def main():
     psi = E.new_state(n, torch.complex128)
     for _ in range(args.warmup):
         fused.run(psi)
-        plain.run(psi)
+        if args.plain_steps > 0:
+            plain.run(psi)
     torch.cuda.synchronize()
 
     # ---- timed region: K fused steps ------------------------------------------
     t_wall = time.perf_counter()
     ms_fused = timed(torch, lambda: fused.run(psi), args.steps, stream)
     wall = time.perf_counter() - t_wall
-    ms_plain = timed(torch, lambda: plain.run(psi), args.plain_steps, stream)
+    ms_plain = timed(torch, lambda: plain.run(psi), args.plain_steps, stream) \
+        if args.plain_steps > 0 else float("nan")
 
     # ---- dominant kernels, HIP events on the launch stream ---------------------
     blk, one = dominant_programs(E, G, n)

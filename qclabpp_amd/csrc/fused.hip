@@ -106,35 +106,23 @@ __global__ __launch_bounds__(256) void k_fused_mfma(typename CT<R>::type* __rest
   for (int e = 0; e < EPL; ++e) eoff[e] = loff | tile_offset(uint32_t(e) << 6, tm);
   __syncthreads();
 
-  // Software pipeline: the next tile's global loads are issued before the
-  // current tile's MFMAs, so their latency overlaps the compute.
-  const uint64_t stride = uint64_t(gridDim.x) * 4;
-  uint64_t w0 = uint64_t(blockIdx.x) * 4;
-  bool act = w0 + wave < ntiles;
-  uint64_t base = act ? deposit_runs(w0 + wave, tm) : 0;
-  C v[EPL];
-  if (act) {
-#pragma unroll
-    for (int e = 0; e < EPL; ++e) v[e] = psi[base + eoff[e]];
-  }
-  while (w0 < ntiles) {
+  for (uint64_t w0 = uint64_t(blockIdx.x) * 4; w0 < ntiles; w0 += uint64_t(gridDim.x) * 4) {
+    const uint64_t w = w0 + wave;
+    const bool act = w < ntiles;
+    const uint64_t base = act ? deposit_runs(w, tm) : 0;
     if (act) {
+      C v[EPL];
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) v[e] = psi[base + eoff[e]];
 #pragma unroll
       for (int e = 0; e < EPL; ++e) tile[wave][lane + 64 * e] = v[e];
     }
     __syncthreads();
-    const uint64_t w0n = w0 + stride;
-    const bool actn = w0n + wave < ntiles;
-    const uint64_t basen = actn ? deposit_runs(w0n + wave, tm) : 0;
-    if (actn) {
-#pragma unroll
-      for (int e = 0; e < EPL; ++e) v[e] = psi[basen + eoff[e]];
-    }
     R b[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      const C x = tile[wave][lA[2 * s + (lane >> 5)] + lg];
-      b[s] = (lane & 16) ? x.y : x.x;
+      const C v = tile[wave][lA[2 * s + (lane >> 5)] + lg];
+      b[s] = (lane & 16) ? v.y : v.x;
     }
     Acc acc[MT];
 #pragma unroll
@@ -157,10 +145,6 @@ __global__ __launch_bounds__(256) void k_fused_mfma(typename CT<R>::type* __rest
 #pragma unroll
       for (int e = 0; e < EPL; ++e) psi[base + eoff[e]] = tile[wave][lane + 64 * e];
     }
-    __syncthreads();  // the tile is reused by the next iteration's writes
-    w0 = w0n;
-    act = actn;
-    base = basen;
   }
 }
 
