@@ -288,6 +288,13 @@ int qsv_dev_max_abs_diff(const void* a, const void* b, uint64_t count, int dtype
  * the library between calls (qsv_release_cache frees it).                  */
 int qsv_host_apply_gates(void* host, int n, int dtype, const qsv_gate* gates,
                          size_t ngates);
+/* Same, running the gate list as one compiled program with fusion width
+ * fuse_k (0 = plain launches).                                            */
+int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_t ngates,
+                      int fuse_k);
+/* Device-resident variant: compile, run on `psi`, synchronise `stream`.   */
+int qsv_dev_simulate(void* psi, int n, int dtype, const qsv_gate* gates, size_t ngates,
+                     int fuse_k, void* stream);
 int qsv_release_cache(void);
 
 #ifdef __cplusplus

@@ -141,6 +141,8 @@ SIGNATURES = {
     "qsv_program_op": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP]),
     "qsv_program_destroy": (_I, [_P]),
     "qsv_release_cache": (_I, []),
+    "qsv_host_simulate": (_I, [_P, _I, _I, _GP, _SZ, _I]),
+    "qsv_dev_simulate": (_I, [_P, _I, _I, _GP, _SZ, _I, _P]),
     "qsv_dev_expectation": (_I, [_P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _DP, _DP, _P]),
     "qsv_dev_inner": (_I, [_P, _P, _I, _I, _DP, _DP, _P]),
     "qsv_dev_apply_pauli_sum": (_I, [_P, _P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _P]),

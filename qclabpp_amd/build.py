@@ -69,5 +69,28 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
+CPP_TESTS = ROOT / "tests" / "cpp"
+CPP_TEST_BIN = CPP_TESTS / "_bin" / "dropin_test"
+
+
+def build_cpp_tests(verbose: bool = True) -> Path:
+    """Compile the C++ drop-in test program against the built library.
+
+    The binary finds libqclabpp_amd.so through a relative rpath, so the tree
+    can be copied elsewhere and still run it.
+    """
+    CPP_TEST_BIN.parent.mkdir(parents=True, exist_ok=True)
+    cmd = [_hipcc(), "-std=c++17", "-O1", f"-I{ROOT / 'include'}",
+           str(CPP_TESTS / "dropin_test.cpp"), "-o", str(CPP_TEST_BIN),
+           f"-L{PKG}", "-lqclabpp_amd", "-Wl,-rpath,$ORIGIN/../../../qclabpp_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"C++ test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {CPP_TEST_BIN}", file=sys.stderr)
+    return CPP_TEST_BIN
+
+
 if __name__ == "__main__":
     build()
+    build_cpp_tests()

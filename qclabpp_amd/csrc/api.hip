@@ -40,6 +40,26 @@ struct Cache {
   size_t bytes = 0;
 } g_cache;
 
+// Grows the cache to `bytes` (caller holds g_cache.mu).
+int reserve_cache(size_t bytes) {
+  if (g_cache.bytes >= bytes) return QSV_OK;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    return fail(QSV_ENODEV, "no GPU visible to the HIP runtime");
+  }
+  if (g_cache.ptr) QSV_CHECK_HIP(hipFree(g_cache.ptr));
+  g_cache.ptr = nullptr;
+  g_cache.bytes = 0;
+  if (hipMalloc(&g_cache.ptr, bytes) != hipSuccess) {
+    g_cache.ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(QSV_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  g_cache.bytes = bytes;
+  return QSV_OK;
+}
+
 }  // namespace
 }  // namespace qsv
 
@@ -172,19 +192,38 @@ int qsv_host_apply_gates(void* host, int n, int dtype, const qsv_gate* gates, si
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, ops[i]));
   const size_t bytes = (size_t(1) << n) * elem_bytes(dtype);
   std::lock_guard<std::mutex> lk(g_cache.mu);
-  if (g_cache.bytes < bytes) {
-    if (g_cache.ptr) QSV_CHECK_HIP(hipFree(g_cache.ptr));
-    g_cache.ptr = nullptr;
-    g_cache.bytes = 0;
-    if (hipMalloc(&g_cache.ptr, bytes) != hipSuccess) {
-      g_cache.ptr = nullptr;
-      (void)hipGetLastError();
-      return fail(QSV_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
-    }
-    g_cache.bytes = bytes;
-  }
+  QSV_TRY(reserve_cache(bytes));
   QSV_CHECK_HIP(hipMemcpy(g_cache.ptr, host, bytes, hipMemcpyHostToDevice));
   for (const KOp& op : ops) QSV_TRY(apply_lowered(op, g_cache.ptr, n, dtype, nullptr));
+  QSV_CHECK_HIP(hipMemcpy(host, g_cache.ptr, bytes, hipMemcpyDeviceToHost));
+  return QSV_OK;
+}
+
+int qsv_dev_simulate(void* psi, int n, int dtype, const qsv_gate* gates, size_t ngates,
+                     int fuse_k, void* stream) {
+  QSV_TRY(check_state_args(psi, n, dtype));
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  if (fuse_k < 0 || fuse_k > kMaxFuse) return fail(QSV_EINVAL, "fuse_k must be in [0, 5]");
+  Program prog;
+  QSV_TRY(prog.build(n, dtype, gates, ngates, fuse_k, 0));
+  auto st = static_cast<hipStream_t>(stream);
+  QSV_TRY(prog.run(psi, st));
+  QSV_CHECK_HIP(hipStreamSynchronize(st));  // the program's coefficients are freed on return
+  return QSV_OK;
+}
+
+int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_t ngates,
+                      int fuse_k) {
+  QSV_TRY(check_state_args(host, n, dtype));
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  if (fuse_k < 0 || fuse_k > kMaxFuse) return fail(QSV_EINVAL, "fuse_k must be in [0, 5]");
+  Program prog;
+  QSV_TRY(prog.build(n, dtype, gates, ngates, fuse_k, 0));
+  const size_t bytes = (size_t(1) << n) * elem_bytes(dtype);
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  QSV_TRY(reserve_cache(bytes));
+  QSV_CHECK_HIP(hipMemcpy(g_cache.ptr, host, bytes, hipMemcpyHostToDevice));
+  QSV_TRY(prog.run(g_cache.ptr, nullptr));
   QSV_CHECK_HIP(hipMemcpy(host, g_cache.ptr, bytes, hipMemcpyDeviceToHost));
   return QSV_OK;
 }

@@ -1,0 +1,362 @@
+// Tests of the QCLAB++-compatible C++ classes (include/qclab) running on the
+// MI355X engine.  Each gate class is applied with Op N/T/C at several qubit
+// placements and offsets, and the result is compared with the dense product
+// op(G) * v built from the class's own matrix() on the host.  Circuits add
+// nesting, pointer gates, [Conj]Trans reversal, matrix() columns, device
+// vectors and QASM round trips (mirroring test/QCircuit.cpp and
+// test/qgates/*.cpp of the reference).
+//
+//   dropin_test          every check (needs a GPU)
+//   dropin_test --host   only the checks that do not touch the GPU
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <sstream>
+#include <string>
+
+#include "qclab/QCircuit.hpp"
+#include "qclab/io/QASMFile.hpp"
+#include "qclab/qgates/CNOT.hpp"
+#include "qclab/qgates/PointerGate1.hpp"
+#include "qclab/qgates/PointerGate2.hpp"
+#include "qclab/qgates/QGate2.hpp"
+
+using namespace qclab;
+using namespace qclab::qgates;
+
+static int g_checks = 0, g_fail = 0;
+#define CHECK(cond, ...)                                              \
+  do {                                                                \
+    ++g_checks;                                                       \
+    if (!(cond)) {                                                    \
+      ++g_fail;                                                       \
+      std::printf("FAIL %s:%d: ", __FILE__, __LINE__);                \
+      std::printf(__VA_ARGS__);                                       \
+      std::printf("\n");                                              \
+    }                                                                 \
+  } while (0)
+
+template <typename T>
+double tol() { return std::is_same_v<T, std::complex<double>> ? 1e-12 : 2e-5; }
+
+template <typename T>
+std::vector<T> random_state(int n, unsigned seed) {
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> nd;
+  std::vector<T> v(size_t(1) << n);
+  double s = 0;
+  for (auto& x : v) { x = T(nd(rng), nd(rng)); s += std::norm(std::complex<double>(x)); }
+  for (auto& x : v) x /= real_t<T>(std::sqrt(s));
+  return v;
+}
+
+template <typename T>
+double max_diff(const std::vector<T>& a, const std::vector<T>& b) {
+  double m = 0;
+  for (size_t i = 0; i < a.size(); ++i) m = std::max(m, double(std::abs(a[i] - b[i])));
+  return m;
+}
+
+// op(G) * v on the host from the object's own matrix
+template <typename T>
+std::vector<T> host_apply(const QObject<T>& g, Op op, int n, const std::vector<T>& v, int offset) {
+  dense::SquareMatrix<T> M = g.matrix_in(n, offset);
+  dense::operateInPlace(op, M);
+  std::vector<T> out(v.size(), T(0));
+  for (int64_t c = 0; c < M.cols(); ++c)
+    for (int64_t r = 0; r < M.rows(); ++r) out[r] += M(r, c) * v[c];
+  return out;
+}
+
+template <typename T>
+std::vector<std::unique_ptr<QObject<T>>> every_gate(int a, int b) {
+  using R = real_t<T>;
+  std::vector<std::unique_ptr<QObject<T>>> g;
+  const R th = R(0.7311), c = std::cos(R(0.4)), s = std::sin(R(0.4));
+  g.push_back(std::make_unique<Identity<T>>(a));
+  g.push_back(std::make_unique<Hadamard<T>>(a));
+  g.push_back(std::make_unique<PauliX<T>>(a));
+  g.push_back(std::make_unique<PauliY<T>>(a));
+  g.push_back(std::make_unique<PauliZ<T>>(a));
+  g.push_back(std::make_unique<Phase45<T>>(a));
+  g.push_back(std::make_unique<Phase90<T>>(a));
+  g.push_back(std::make_unique<RotationX<T>>(a, th));
+  g.push_back(std::make_unique<RotationY<T>>(a, c, s));
+  g.push_back(std::make_unique<RotationZ<T>>(a, -th));
+  g.push_back(std::make_unique<Phase<T>>(a, th));
+  g.push_back(std::make_unique<Phase<T>>(a, c, s));
+  for (int cs = 0; cs < 2; ++cs) {
+    g.push_back(std::make_unique<CX<T>>(a, b, cs));
+    g.push_back(std::make_unique<CNOT<T>>(b, a, cs));
+    g.push_back(std::make_unique<CY<T>>(a, b, cs));
+    g.push_back(std::make_unique<CZ<T>>(b, a, cs));
+    g.push_back(std::make_unique<CPhase<T>>(a, b, th, cs));
+    g.push_back(std::make_unique<CRotationX<T>>(b, a, th, cs));
+    g.push_back(std::make_unique<CRotationY<T>>(a, b, c, s, cs));
+    g.push_back(std::make_unique<CRotationZ<T>>(b, a, -th, cs));
+  }
+  g.push_back(std::make_unique<SWAP<T>>(a, b));
+  g.push_back(std::make_unique<iSWAP<T>>(b, a));
+  g.push_back(std::make_unique<RotationXX<T>>(a, b, th));
+  g.push_back(std::make_unique<RotationYY<T>>(b, a, th));
+  g.push_back(std::make_unique<RotationZZ<T>>(a, b, c, s));
+  return g;
+}
+
+// a user gate defined only by its matrix (dense 2x2 path)
+template <typename T>
+class MyGate1 : public QGate1<T> {
+ public:
+  MyGate1(int q) : QGate1<T>(q) {}
+  bool fixed() const override { return true; }
+  dense::SquareMatrix<T> matrix() const override {
+    return dense::SquareMatrix<T>(T(0.6, 0.0), T(0.0, 0.8), T(0.0, 0.8), T(0.6, 0.0));
+  }
+  int toQASM(std::ostream&, const int = 0) const override { return -1; }
+
+ protected:
+  bool equals(const QObject<T>& o) const override { return dynamic_cast<const MyGate1<T>*>(&o); }
+};
+
+// a user 2-qubit gate with a matrix that is symmetric under exchanging its
+// qubits (where QGate2's vector convention and the kron order agree)
+template <typename T>
+class MyGate2 : public QGate2<T> {
+ public:
+  MyGate2(int a, int b) : q_{std::min(a, b), std::max(a, b)} {}
+  bool fixed() const override { return true; }
+  bool controlled() const override { return false; }
+  int qubit() const override { return q_[0]; }
+  std::vector<int> qubits() const override { return {q_[0], q_[1]}; }
+  void setQubits(const int* q) override { q_[0] = q[0]; q_[1] = q[1]; }
+  dense::SquareMatrix<T> matrix() const override {
+    const T a(0.5, 0.5), b(0.5, -0.5), o(0);
+    return dense::SquareMatrix<T>(a, o, o, b, o, a, b, o, o, b, a, o, b, o, o, a);
+  }
+  int toQASM(std::ostream&, const int = 0) const override { return -1; }
+
+ protected:
+  bool equals(const QObject<T>& o) const override { return dynamic_cast<const MyGate2<T>*>(&o); }
+  int q_[2];
+};
+
+template <typename T>
+void gate_checks(const char* tn) {
+  const int n = 6;
+  const Op ops[3] = {Op::NoTrans, Op::Trans, Op::ConjTrans};
+  const int placements[][3] = {{0, 1, 0}, {2, 5, 0}, {4, 1, 0}, {0, 3, 2}, {3, 0, 1}};
+  unsigned seed = 1;
+  for (const auto& pl : placements) {
+    auto gates = every_gate<T>(pl[0], pl[1]);
+    gates.push_back(std::make_unique<MyGate1<T>>(pl[0]));
+    gates.push_back(std::make_unique<MyGate2<T>>(pl[0], pl[1]));
+    for (const auto& g : gates)
+      for (Op op : ops) {
+        auto v = random_state<T>(n, seed++);
+        const auto want = host_apply(*g, op, n, v, pl[2]);
+        g->apply(op, n, v, pl[2]);
+        const double d = max_diff(v, want);
+        CHECK(d < tol<T>(), "%s gate qubits %d,%d offset %d op %c: max diff %.3e", tn, pl[0], pl[1],
+              pl[2], char(op), d);
+      }
+  }
+}
+
+template <typename T>
+QCircuit<T> sample_circuit(int n, unsigned seed) {
+  using R = real_t<T>;
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> ang(0, 2 * M_PI);
+  QCircuit<T> c(n);
+  for (int layer = 0; layer < 4; ++layer) {
+    for (int q = 0; q < n; ++q) {
+      const int k = int(rng() % 3);
+      if (k == 0) c.push_back(std::make_unique<RotationX<T>>(q, R(ang(rng))));
+      if (k == 1) c.push_back(std::make_unique<RotationY<T>>(q, R(ang(rng))));
+      if (k == 2) c.push_back(std::make_unique<Phase<T>>(q, R(ang(rng))));
+    }
+    for (int q = layer % 2; q + 1 < n; q += 2) c.push_back(std::make_unique<CX<T>>(q, q + 1));
+    c.push_back(std::make_unique<RotationZZ<T>>(0, n - 1, R(ang(rng))));
+    c.push_back(std::make_unique<CPhase<T>>(n - 1, 1, R(ang(rng)), 0));
+    c.push_back(std::make_unique<iSWAP<T>>(1, 2));
+  }
+  return c;
+}
+
+template <typename T>
+void circuit_checks(const char* tn) {
+  const int n = 7;
+  // whole circuit against its own matrix; ConjTrans undoes it
+  QCircuit<T> c = sample_circuit<T>(n, 11);
+  const auto v0 = random_state<T>(n, 99);
+  auto v = v0;
+  c.simulate(v);
+  const auto want = host_apply<T>(c, Op::NoTrans, n, v0, 0);
+  CHECK(max_diff(v, want) < 10 * tol<T>(), "%s circuit simulate vs matrix: %.3e", tn, max_diff(v, want));
+  c.apply(Op::ConjTrans, n, v);
+  CHECK(max_diff(v, v0) < 10 * tol<T>(), "%s ConjTrans round trip: %.3e", tn, max_diff(v, v0));
+  auto vt = v0;
+  c.apply(Op::Trans, n, vt);
+  const auto want_t = host_apply<T>(c, Op::Trans, n, v0, 0);
+  CHECK(max_diff(vt, want_t) < 10 * tol<T>(), "%s Trans: %.3e", tn, max_diff(vt, want_t));
+
+  // nested circuit with an offset inside a larger register, plus pointer gates
+  QCircuit<T> outer(n + 2);
+  auto inner = std::make_unique<QCircuit<T>>(3, 2);
+  inner->push_back(std::make_unique<Hadamard<T>>(0));
+  inner->push_back(std::make_unique<CX<T>>(0, 2));
+  inner->push_back(std::make_unique<RotationY<T>>(1, real_t<T>(0.3)));
+  outer.push_back(std::move(inner));
+  Hadamard<T> h(1);
+  RotationXX<T> rxx(0, 2, real_t<T>(1.1));
+  outer.push_back(std::make_unique<PointerGate1<T>>(&h, 3));
+  outer.push_back(std::make_unique<PointerGate2<T>>(&rxx, 4));
+  outer.push_back(std::make_unique<CZ<T>>(8, 0));
+  const auto w0 = random_state<T>(n + 2, 5);
+  for (int off = 0; off < 2; ++off) {
+    auto w = random_state<T>(n + 3, 6);
+    const auto w_in = w;
+    outer.apply(Op::NoTrans, n + 3, w, off);
+    const auto want_w = host_apply<T>(outer, Op::NoTrans, n + 3, w_in, off);
+    CHECK(max_diff(w, want_w) < 10 * tol<T>(), "%s nested/pointer offset %d: %.3e", tn, off,
+          max_diff(w, want_w));
+  }
+  // matrix columns: simulate(e_j) is column j of matrix()
+  QCircuit<T> small = sample_circuit<T>(4, 3);
+  const auto M = small.matrix();
+  double worst = 0;
+  for (int j = 0; j < 16; ++j) {
+    std::vector<T> e(16, T(0));
+    e[j] = T(1);
+    small.simulate(e);
+    for (int r = 0; r < 16; ++r) worst = std::max(worst, double(std::abs(e[r] - M(r, j))));
+  }
+  CHECK(worst < 10 * tol<T>(), "%s simulate(e_j) vs matrix(): %.3e", tn, worst);
+
+  // device-resident vector
+  auto host = v0;
+  c.simulate(host);
+  T* d = nullptr;
+  const size_t bytes = v0.size() * sizeof(T);
+  CHECK(hipMalloc(&d, bytes) == hipSuccess, "hipMalloc");
+  CHECK(hipMemcpy(d, v0.data(), bytes, hipMemcpyHostToDevice) == hipSuccess, "upload");
+  c.simulate_device(d);
+  std::vector<T> back(v0.size());
+  CHECK(hipMemcpy(back.data(), d, bytes, hipMemcpyDeviceToHost) == hipSuccess, "download");
+  (void)hipFree(d);
+  CHECK(max_diff(back, host) < tol<T>(), "%s simulate_device vs simulate: %.3e", tn, max_diff(back, host));
+
+  // QASM round trip through the extended reader (no Phase: it is written as rz)
+  QCircuit<T> q(5);
+  q.push_back(std::make_unique<Hadamard<T>>(0));
+  q.push_back(std::make_unique<RotationX<T>>(1, real_t<T>(0.25)));
+  q.push_back(std::make_unique<CPhase<T>>(0, 3, real_t<T>(-1.5)));
+  q.push_back(std::make_unique<CRotationY<T>>(4, 2, real_t<T>(2.0), 0));
+  q.push_back(std::make_unique<RotationZZ<T>>(1, 4, real_t<T>(0.5)));
+  q.push_back(std::make_unique<iSWAP<T>>(2, 3));
+  q.push_back(std::make_unique<Phase45<T>>(4));
+  q.push_back(std::make_unique<Phase90<T>>(3));
+  q.push_back(std::make_unique<SWAP<T>>(0, 4));
+  std::ostringstream text;
+  text << "OPENQASM 2.0;\ninclude \"qelib1.inc\";\nqreg q[5];\n";
+  CHECK(q.toQASM(text) == 0, "toQASM");
+  auto f = io::QASMFile::from_string(text.str(), io::QASMFile::Dialect::Extended);
+  CHECK(f.error().empty() && f.nbQubits() == 5, "extended parse: %s", f.error().c_str());
+  QCircuit<T> q2(5);
+  for (auto& g : f.gates<QObject<T>>()) q2.push_back(std::move(g));
+  CHECK(q2.nbGates() == q.nbGates(), "%s QASM round trip gate count %zu", tn, q2.nbGates());
+  auto a = random_state<T>(5, 8), b = a;
+  q.simulate(a);
+  q2.simulate(b);
+  CHECK(max_diff(a, b) < 1e-6, "%s QASM round trip state: %.3e", tn, max_diff(a, b));
+}
+
+// checks with no GPU work: headers, QASM readers, matrices
+template <typename T>
+void host_checks(const char* tn) {
+  // the reference dialect: numeric angles read, `pi` stops the list
+  const std::string text =
+      "OPENQASM 2.0;\ninclude \"qelib1.inc\";\nqreg q[4];\ncreg c[4];\n"
+      "h q[0];\nrx(0.5) q[1];\ncx q[1], q[3];\n// a comment\nswap q[0],q[2]; p(1.25) q[2];\n"
+      "s q[1];\nrz(pi/2) q[0];\nx q[3];\n";
+  std::streambuf* old = std::cerr.rdbuf();
+  std::ostringstream err;
+  std::cerr.rdbuf(err.rdbuf());
+  auto ref = io::QASMFile::from_string(text);
+  auto gates = ref.gates<QObject<T>>();
+  std::cerr.rdbuf(old);
+  CHECK(ref.nbQubits() == 4, "%s reference dialect qubits", tn);
+  CHECK(gates.size() == 5, "%s reference dialect keeps the 5 gates before rz(pi/2), got %zu", tn, gates.size());
+  CHECK(err.str().find("Failed parsing QASM file!") != std::string::npos, "%s truncation message", tn);
+  if (gates.size() == 5) {
+    CHECK(*gates[1] == RotationX<T>(1, real_t<T>(0.5)), "%s rx parsed", tn);
+    CHECK(dynamic_cast<const Phase<T>*>(gates[4].get()) != nullptr, "%s p parsed", tn);
+  }
+  // the extended dialect reads all of it, `pi` included
+  std::cerr.rdbuf(err.rdbuf());
+  auto ext = io::QASMFile::from_string(text, io::QASMFile::Dialect::Extended);
+  std::cerr.rdbuf(old);
+  auto eg = ext.gates<QObject<T>>();
+  CHECK(eg.size() == 8 && ext.error().empty(), "%s extended dialect: %zu gates, '%s'", tn, eg.size(),
+        ext.error().c_str());
+  if (eg.size() == 8) {
+    const auto* rz = dynamic_cast<const RotationZ<T>*>(eg[6].get());
+    CHECK(rz && std::abs(double(rz->theta()) - M_PI / 2) < 1e-6, "%s rz(pi/2)", tn);
+  }
+  auto bc = io::QASMFile::from_string("qreg a[2]; qreg b[3]; h b; cx a[1], b[0]; rzz(-pi/4*2) a[0],b[2];",
+                                      io::QASMFile::Dialect::Extended);
+  CHECK(bc.nbQubits() == 5 && bc.gates<QObject<T>>().size() == 5, "%s broadcast/multi-register", tn);
+
+  // toQASM formats (include/qclab/qasm.hpp of the reference)
+  std::ostringstream s1;
+  CX<T>(2, 0, 0).toQASM(s1, 1);
+  CHECK(s1.str() == "x q[3];\ncx q[3], q[1];\nx q[3];\n", "%s CX controlState 0 QASM: '%s'", tn, s1.str().c_str());
+  std::ostringstream s2;
+  Phase<T>(1, real_t<T>(0.5)).toQASM(s2);
+  CHECK(s2.str().rfind("rz(0.5", 0) == 0, "%s Phase written as rz: '%s'", tn, s2.str().c_str());
+
+  // matrices and equality semantics
+  CHECK(CX<T>(0, 1).matrix() == dense::SquareMatrix<T>(1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0),
+        "%s CX(0,1) matrix", tn);
+  CHECK(CX<T>(1, 0).matrix() == dense::SquareMatrix<T>(1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0),
+        "%s CX(1,0) matrix", tn);
+  CHECK(CX<T>(0, 1) == CX<T>(2, 5) && CX<T>(0, 1) != CX<T>(1, 0), "%s CX equality", tn);
+  RotationX<T> r1(0, real_t<T>(0.3)), r2(0, real_t<T>(0.4));
+  CHECK(std::abs(double((r1 * r2).theta()) - 0.7) < 1e-6, "%s rotation product", tn);
+  QCircuit<T> c1(2), c2(2);
+  c1.push_back(std::make_unique<CZ<T>>(0, 1));
+  c2.push_back(std::make_unique<CZ<T>>(1, 0));
+  CHECK(c1 == c2 && nrmF(c1, c2) == 0.0, "%s circuit equality by matrix", tn);
+  CHECK(!c1.canInsert(Hadamard<T>(2)) && c1.canInsert(Hadamard<T>(1)), "%s canInsert", tn);
+}
+
+int main(int argc, char** argv) {
+  const bool host_only = argc > 1 && std::strcmp(argv[1], "--host") == 0;
+  host_checks<std::complex<double>>("c128");
+  host_checks<std::complex<float>>("c64");
+  if (!host_only) {
+    if (qsv_device_count() < 1) {
+      std::printf("FAIL: no GPU visible (run with --host for the host checks)\n");
+      return 2;
+    }
+    gate_checks<std::complex<double>>("c128");
+    gate_checks<std::complex<float>>("c64");
+    circuit_checks<std::complex<double>>("c128");
+    circuit_checks<std::complex<float>>("c64");
+    bool threw = false;
+    try {
+      std::vector<std::complex<double>> bad(8);
+      Hadamard<std::complex<double>>(5).apply(Op::NoTrans, 3, bad);
+    } catch (const backend::error& e) {
+      threw = e.code() == QSV_EQUBIT;
+    }
+    CHECK(threw, "an out-of-range qubit raises backend::error(QSV_EQUBIT)");
+  }
+  std::printf("%s: %d checks, %d failed\n", host_only ? "host" : "all", g_checks, g_fail);
+  return g_fail ? 1 : 0;
+}

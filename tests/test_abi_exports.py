@@ -1,0 +1,51 @@
+"""The C-ABI library loads without a GPU and exports what include/qsv.h declares."""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+
+import pytest
+
+from qclabpp_amd import _lib
+
+
+def _loaded():
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("engine library not built")
+    return _lib.lib()
+
+
+def test_every_declared_function_is_exported():
+    lib = _loaded()
+    declared = _lib.header_functions()
+    assert len(declared) >= 25
+    missing = [name for name in declared if not hasattr(lib, name)]
+    assert not missing, f"declared in include/qsv.h but not exported: {missing}"
+
+
+def test_python_signatures_cover_the_header():
+    """Every declared function has a ctypes signature, and no stale ones."""
+    assert sorted(_lib.SIGNATURES) == _lib.header_functions()
+
+
+def test_exports_are_plain_c_symbols():
+    """No C++ mangling on the boundary (nm lists the names verbatim)."""
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True,
+                         text=True)
+    if out.returncode != 0:
+        pytest.skip("nm unavailable")
+    names = {line.split()[-1] for line in out.stdout.splitlines() if line.strip()}
+    for name in _lib.header_functions():
+        assert name in names, name
+
+
+def test_calls_that_need_no_gpu():
+    lib = _loaded()
+    assert b"gfx950" in lib.qsv_version()
+    assert lib.qsv_device_count() >= 0
+    # argument checks fail before any device work
+    st = lib.qsv_host_simulate(None, 3, 2, None, 0, 0)
+    assert st == _lib.QSV_EINVAL
+    assert b"null" in lib.qsv_last_error()
+    buf = (ctypes.c_double * 16)()
+    assert lib.qsv_host_simulate(buf, 3, 9, None, 0, 0) == 5  # QSV_EDTYPE
