@@ -42,6 +42,10 @@ def parse():
     ap.add_argument("--batch", type=int, default=512,
                     help="circuits of the batched HEA sweep (0 skips it)")
     ap.add_argument("--batch-runs", type=int, default=5)
+    ap.add_argument("--precision-n", type=int, default=28,
+                    help="qubits of the complex64-vs-complex128 comparison (0 skips it)")
+    ap.add_argument("--precision-big-n", type=int, default=33,
+                    help="qubits of the complex64-only run (64 GiB at 33; 0 skips it)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the dominant kernels and a 16 GiB copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -172,6 +176,66 @@ def batch_leg(args):
         out["cpu_reference"] = {"circuits_per_s": len(sample) / secs, "sample": f"first {len(sample)} "
                                 "circuits, QCircuit::simulate (reference, complex<float>)",
                                 "cores": oracle.ref_threads() if oracle.ref_available() else None}
+    return out
+
+
+def precision_leg(args):
+    """complex64 against complex128 on the seeded depth-20 brickwork (seed
+    20261203), both run with the fused program from |0...0>.  At n=28 the
+    complex64 state is compared with the complex128 one: max |delta amp|,
+    norm drift |<psi|psi> - 1| and the error of <Z_i Z_{i+1}> (i = 0..n-2)
+    and of their sum, every reduction accumulated in FP64.  At n=33 (64 GiB)
+    only complex64 fits; its FP64 norm drift, <Z_i Z_{i+1}> and run time are
+    reported."""
+    import numpy as np
+    import torch
+
+    from qclabpp_amd import engine as E
+    from qclabpp_amd import observables as O
+    from qclabpp_amd import workloads as W
+
+    out = {}
+    stream = torch.cuda.current_stream()
+    if args.precision_n > 0:
+        n = args.precision_n
+        circ = W.brickwork(n, 20, seed=W.SEED_PREC)
+        zz = O.zz_chain(n)
+        res = {}
+        states = {}
+        for dt, tdt in (("c128", torch.complex128), ("c64", torch.complex64)):
+            prog = E.Program(n, circ, fuse_k=5, dtype=dt)
+            psi = E.new_state(n, tdt)
+            ms = timed(torch, lambda: prog.run(psi), 1, stream)
+            vals, tot = O.expectation(psi, zz)
+            res[dt] = {"ms": ms, "norm_drift": abs(O.norm2(psi) - 1.0), "zz": vals, "zz_sum": tot}
+            states[dt] = psi
+        d = (states["c64"].to(torch.complex128) - states["c128"]).abs().max().item()
+        del states
+        torch.cuda.empty_cache()
+        out["n28"] = {
+            "workload": f"brickwork n={n} depth=20 ({len(circ)} gates), fused k<=5",
+            "max_abs_amp_c64_vs_c128": d,
+            "norm_drift_c64": res["c64"]["norm_drift"], "norm_drift_c128": res["c128"]["norm_drift"],
+            "max_abs_zz_err_c64": float(np.abs(res["c64"]["zz"] - res["c128"]["zz"]).max()),
+            "zz_sum_c128": res["c128"]["zz_sum"],
+            "abs_zz_sum_err_c64": abs(res["c64"]["zz_sum"] - res["c128"]["zz_sum"]),
+            "ms_c64": res["c64"]["ms"], "ms_c128": res["c128"]["ms"]}
+    if args.precision_big_n > 0:
+        n = args.precision_big_n
+        circ = W.brickwork(n, 20, seed=W.SEED_PREC)
+        prog = E.Program(n, circ, fuse_k=5, dtype="c64")
+        psi = E.new_state(n, torch.complex64)
+        ms = timed(torch, lambda: prog.run(psi), 1, stream)
+        vals, tot = O.expectation(psi, O.zz_chain(n))
+        info = prog.info()
+        out["n33"] = {
+            "workload": f"brickwork n={n} depth=20 ({len(circ)} gates), complex64, fused k<=5",
+            "state_bytes": (1 << n) * 8, "ms": ms, "gate_passes_per_s": len(circ) / (ms / 1e3),
+            "hbm_GBps": info["alg_bytes"] / (ms / 1e3) / 1e9, "launches": info["nlaunch"],
+            "norm_drift_fp64": abs(O.norm2(psi) - 1.0),
+            "zz": [float(v) for v in vals], "zz_sum": tot}
+        del psi
+        torch.cuda.empty_cache()
     return out
 
 
@@ -315,6 +379,8 @@ def main():
     if args.batch > 0:
         out["batch"] = batch_leg(args)
         torch.cuda.empty_cache()
+    if args.precision_n > 0 or args.precision_big_n > 0:
+        out["precision"] = precision_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, circ, args.cpu_gates)
     print(json.dumps(out))
