@@ -114,7 +114,7 @@ def apply(n: int, gates, psi: np.ndarray, op: str = "N") -> np.ndarray:
     return out
 
 
-def apply_gate(n: int, gate, psi: np.ndarray, op: str = "N") -> np.ndarray:
+def apply_gate(n: int, gate, psi: np.ndarray, op: str | None = None) -> np.ndarray:
     out = np.ascontiguousarray(psi).copy()
     rec = gate.record(op)
     st = oracle_lib().oracle_apply_gate(n, _dtype_code(out), C.byref(rec), _ptr(out))
@@ -123,7 +123,7 @@ def apply_gate(n: int, gate, psi: np.ndarray, op: str = "N") -> np.ndarray:
     return out
 
 
-def ref_apply_gate(n: int, gate, psi: np.ndarray, op: str = "N") -> np.ndarray:
+def ref_apply_gate(n: int, gate, psi: np.ndarray, op: str | None = None) -> np.ndarray:
     out = np.ascontiguousarray(psi).copy()
     rec = gate.record(op)
     st = ref_lib().ref_apply_gate(n, _dtype_code(out), C.byref(rec), _ptr(out))

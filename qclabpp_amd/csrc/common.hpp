@@ -39,6 +39,17 @@ struct Status {
     if (s_ != QSV_OK) return s_;                                              \
   } while (0)
 
+// A launch may hold fewer than 2^32 work-items (gridDim.x * blockDim.x), so
+// sweeps over more blocks than this run as several launches over consecutive
+// block ranges; each such kernel takes the index of its first block (blk0).
+constexpr uint64_t kMaxLaunchBlocks = uint64_t(1) << 22;
+
+#define QSV_LAUNCH_BLOCKS(KERNEL, NBLOCKS, THREADS, LDS, STREAM, ...)                           \
+  for (uint64_t b0_ = 0, nb_ = (NBLOCKS); b0_ < nb_; b0_ += ::qsv::kMaxLaunchBlocks)           \
+  hipLaunchKernelGGL(KERNEL, dim3(unsigned(nb_ - b0_ < ::qsv::kMaxLaunchBlocks ? nb_ - b0_       \
+                                                                      : ::qsv::kMaxLaunchBlocks)), \
+                     dim3(THREADS), LDS, STREAM, __VA_ARGS__, b0_)
+
 inline int fail(int code, const std::string& msg) {
   set_error(msg);
   return code;

@@ -185,13 +185,13 @@ __global__ __launch_bounds__(64) void k_densek_simple(typename CT<R>::type* __re
 template <typename R, int NP>
 __global__ __launch_bounds__(256) void k_diagk(typename CT<R>::type* __restrict__ psi, uint64_t nel,
                                                const typename CT<R>::type* __restrict__ table,
-                                               int k, BlockBits bb) {
+                                               int k, BlockBits bb, uint64_t blk0) {
   using C = typename CT<R>::type;
   __shared__ C tab[1 << kMaxDiag];
   const int D = 1 << k;
   for (int i = threadIdx.x; i < D; i += 256) tab[i] = table[i];
   __syncthreads();
-  const uint64_t e0 = uint64_t(blockIdx.x) * (256 * NP) + threadIdx.x;
+  const uint64_t e0 = (blk0 + blockIdx.x) * (256 * NP) + threadIdx.x;
   C x[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -269,8 +269,8 @@ int launch_fused(const KOp& op, const DeviceBlock& db, void* psi, int n, hipStre
   if (op.kind == K_DIAGK) {
     constexpr int NP = 4;
     const uint64_t blocks = (N + 256 * NP - 1) / (256 * NP);
-    hipLaunchKernelGGL((k_diagk<R, NP>), dim3(unsigned(blocks)), dim3(256), 0, st,
-                       static_cast<C*>(psi), N, static_cast<const C*>(db.table), k, bb);
+    QSV_LAUNCH_BLOCKS((k_diagk<R, NP>), blocks, 256, 0, st, static_cast<C*>(psi), N,
+                      static_cast<const C*>(db.table), k, bb);
   } else if (op.kind == K_DENSEK) {
     if (k >= 3 && n >= k + 6) {
       const TileMap tm = make_tile_map(std::vector<int>(op.q, op.q + k), n);

@@ -135,7 +135,8 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_expect(const C* __restrict
 template <typename C>
 __global__ __launch_bounds__(256) void k_pauli_apply(const C* __restrict__ in, C* __restrict__ out,
                                                      uint64_t N, const DevPauli* __restrict__ terms,
-                                                     const int* __restrict__ gstart, int ngroups) {
+                                                     const int* __restrict__ gstart, int ngroups,
+                                                     uint64_t blk0) {
   extern __shared__ unsigned char smem[];
   DevPauli* st = reinterpret_cast<DevPauli*>(smem);
   int* gs = reinterpret_cast<int*>(st + gstart[ngroups]);
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void k_pauli_apply(const C* __restrict__ in, C
   for (int i = threadIdx.x; i < nterms; i += blockDim.x) st[i] = terms[i];
   for (int i = threadIdx.x; i <= ngroups; i += blockDim.x) gs[i] = gstart[i];
   __syncthreads();
-  const uint64_t x = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t x = (blk0 + blockIdx.x) * 256 + threadIdx.x;
   if (x >= N) return;
   double2 acc = make_double2(0, 0);
   for (int g = 0; g < ngroups; ++g) {
@@ -224,10 +225,9 @@ int launch_pauli_apply(const void* in, void* out, int n, const void* dev_terms,
   if (nterms > kMaxPauliTerms) return fail(QSV_EINVAL, "Pauli sum has too many terms");
   // terms + group table in LDS
   const size_t lds = size_t(nterms) * sizeof(DevPauli) + size_t(ngroups + 1) * sizeof(int);
-  hipLaunchKernelGGL((k_pauli_apply<C>), dim3(unsigned((N + 255) / 256)), dim3(256), lds, st,
-                     static_cast<const C*>(in), static_cast<C*>(out), N,
-                     static_cast<const DevPauli*>(dev_terms), static_cast<const int*>(dev_group_start),
-                     ngroups);
+  QSV_LAUNCH_BLOCKS((k_pauli_apply<C>), (N + 255) / 256, 256, lds, st, static_cast<const C*>(in),
+                    static_cast<C*>(out), N, static_cast<const DevPauli*>(dev_terms),
+                    static_cast<const int*>(dev_group_start), ngroups);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
 }

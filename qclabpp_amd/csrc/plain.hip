@@ -42,9 +42,9 @@ struct M4 {
 // ---- apply2: 2x2 on pairs (a, a|s) -----------------------------------------
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_dense1(C* __restrict__ psi, uint64_t cnt, int b,
-                                                M2<C> m) {
+                                                M2<C> m, uint64_t blk0) {
   const uint64_t s = uint64_t(1) << b;
-  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP], y[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -69,9 +69,9 @@ __global__ __launch_bounds__(TPB) void k_dense1(C* __restrict__ psi, uint64_t cn
 // Same operation for b < 6: partner amplitude lives in lane (lane ^ 2^b).
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_dense1_lanes(C* __restrict__ psi, uint64_t nel, int b,
-                                                      M2<C> m) {
+                                                      M2<C> m, uint64_t blk0) {
   const int s = 1 << b;
-  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  const uint64_t e0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -92,8 +92,8 @@ __global__ __launch_bounds__(TPB) void k_dense1_lanes(C* __restrict__ psi, uint6
 // ---- RotationZ-class diagonal on every amplitude ---------------------------
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_diag1(C* __restrict__ psi, uint64_t nel, int b, C d0,
-                                               C d1) {
-  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+                                               C d1, uint64_t blk0) {
+  const uint64_t e0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -109,9 +109,9 @@ __global__ __launch_bounds__(TPB) void k_diag1(C* __restrict__ psi, uint64_t nel
 
 // ---- apply2b: scale the |1> half ------------------------------------------------
 template <typename C, int NP>
-__global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cnt, int b, C lam) {
+__global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cnt, int b, C lam, uint64_t blk0) {
   const uint64_t s = uint64_t(1) << b;
-  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -129,8 +129,8 @@ __global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cn
 // blo < bhi are the two bit positions; a1 carries the control bit, s the target.
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_cdense1(C* __restrict__ psi, uint64_t cnt, int blo,
-                                                 int bhi, uint64_t a1, uint64_t s, M2<C> m) {
-  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+                                                 int bhi, uint64_t a1, uint64_t s, M2<C> m, uint64_t blk0) {
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP], y[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -156,9 +156,9 @@ __global__ __launch_bounds__(TPB) void k_cdense1(C* __restrict__ psi, uint64_t c
 // the control bit matches.
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_cdense1_lanes(C* __restrict__ psi, uint64_t nel, int bt,
-                                                       int bc, int cs, M2<C> m) {
+                                                       int bc, int cs, M2<C> m, uint64_t blk0) {
   const int s = 1 << bt;
-  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+  const uint64_t e0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -179,8 +179,8 @@ __global__ __launch_bounds__(TPB) void k_cdense1_lanes(C* __restrict__ psi, uint
 // ---- apply4b: scale one quarter ------------------------------------------------
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_cphase(C* __restrict__ psi, uint64_t cnt, int blo,
-                                                int bhi, uint64_t b1, C lam) {
-  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+                                                int bhi, uint64_t b1, C lam, uint64_t blk0) {
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -197,8 +197,8 @@ __global__ __launch_bounds__(TPB) void k_cphase(C* __restrict__ psi, uint64_t cn
 // ---- apply4bc: exchange |01>,|10> and scale both by lam ---------------------
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_swap(C* __restrict__ psi, uint64_t cnt, int blo, int bhi,
-                                              C lam, int scale) {
-  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+                                              C lam, int scale, uint64_t blk0) {
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   const uint64_t sl = uint64_t(1) << blo, sh = uint64_t(1) << bhi;
   C x[NP], y[NP];
 #pragma unroll
@@ -226,8 +226,8 @@ __global__ __launch_bounds__(TPB) void k_swap(C* __restrict__ psi, uint64_t cnt,
 // quad position p = 2*bit(b0) + bit(b1).
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_dense2(C* __restrict__ psi, uint64_t cnt, int b0, int b1,
-                                                M4<C> m) {
-  const uint64_t k0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+                                                M4<C> m, uint64_t blk0) {
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   const uint64_t s0 = uint64_t(1) << b0, s1 = uint64_t(1) << b1;
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -254,8 +254,8 @@ __global__ __launch_bounds__(TPB) void k_dense2(C* __restrict__ psi, uint64_t cn
 // ---- diagonal 4x4 on every amplitude ------------------------------------------
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_diag2(C* __restrict__ psi, uint64_t nel, int b0, int b1,
-                                               C d0, C d1, C d2, C d3) {
-  const uint64_t e0 = uint64_t(blockIdx.x) * (TPB * NP) + threadIdx.x;
+                                               C d0, C d1, C d2, C d3, uint64_t blk0) {
+  const uint64_t e0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -275,8 +275,8 @@ __global__ __launch_bounds__(TPB) void k_diag2(C* __restrict__ psi, uint64_t nel
 
 // ---- psi := |idx> --------------------------------------------------------------
 template <typename C>
-__global__ __launch_bounds__(TPB) void k_set_basis(C* __restrict__ psi, uint64_t nel, uint64_t idx) {
-  const uint64_t e = uint64_t(blockIdx.x) * TPB + threadIdx.x;
+__global__ __launch_bounds__(TPB) void k_set_basis(C* __restrict__ psi, uint64_t nel, uint64_t idx, uint64_t blk0) {
+  const uint64_t e = (blk0 + blockIdx.x) * TPB + threadIdx.x;
   if (e < nel) psi[e] = mk<C>(e == idx ? 1 : 0, 0);
 }
 
@@ -285,8 +285,8 @@ C to_c(const cd& z) {
   return mk<C>(static_cast<decltype(C::x)>(z.real()), static_cast<decltype(C::x)>(z.imag()));
 }
 
-inline unsigned blocks_for(uint64_t items, int per_block) {
-  return unsigned((items + per_block - 1) / per_block);
+inline uint64_t blocks_for(uint64_t items, int per_block) {
+  return (items + per_block - 1) / per_block;
 }
 
 }  // namespace
@@ -306,31 +306,31 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
       const M2<C> m{to_c<C>(op.m[0]), to_c<C>(op.m[1]), to_c<C>(op.m[2]), to_c<C>(op.m[3])};
       const int b = bit(op.q[0]);
       if (b < tu.lane_bits && n >= 6) {
-        hipLaunchKernelGGL((k_dense1_lanes<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st,
+        QSV_LAUNCH_BLOCKS((k_dense1_lanes<C, NP>), blocks_for(N, per), TPB, 0, st,
                            psi, N, b, m);
       } else {
-        hipLaunchKernelGGL((k_dense1<C, NP>), dim3(blocks_for(N / 2, per)), dim3(TPB), 0, st, psi,
+        QSV_LAUNCH_BLOCKS((k_dense1<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi,
                            N / 2, b, m);
       }
       break;
     }
     case K_DIAG1:
-      hipLaunchKernelGGL((k_diag1<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st, psi, N,
+      QSV_LAUNCH_BLOCKS((k_diag1<C, NP>), blocks_for(N, per), TPB, 0, st, psi, N,
                          bit(op.q[0]), to_c<C>(op.m[0]), to_c<C>(op.m[1]));
       break;
     case K_PHASE1:
-      hipLaunchKernelGGL((k_phase1<C, NP>), dim3(blocks_for(N / 2, per)), dim3(TPB), 0, st, psi,
+      QSV_LAUNCH_BLOCKS((k_phase1<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi,
                          N / 2, bit(op.q[0]), to_c<C>(op.m[0]));
       break;
     case K_CDENSE1: {
       const M2<C> m{to_c<C>(op.m[0]), to_c<C>(op.m[1]), to_c<C>(op.m[2]), to_c<C>(op.m[3])};
       const int bc = bit(op.q[0]), bt = bit(op.q[1]);
       if (bt < tu.lane_bits && n >= 6) {
-        hipLaunchKernelGGL((k_cdense1_lanes<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st,
+        QSV_LAUNCH_BLOCKS((k_cdense1_lanes<C, NP>), blocks_for(N, per), TPB, 0, st,
                            psi, N, bt, bc, op.cs, m);
       } else {
         const uint64_t a1 = uint64_t(op.cs) << bc;
-        hipLaunchKernelGGL((k_cdense1<C, NP>), dim3(blocks_for(N / 4, per)), dim3(TPB), 0, st,
+        QSV_LAUNCH_BLOCKS((k_cdense1<C, NP>), blocks_for(N / 4, per), TPB, 0, st,
                            psi, N / 4, std::min(bc, bt), std::max(bc, bt), a1,
                            uint64_t(1) << bt, m);
       }
@@ -339,26 +339,26 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
     case K_CPHASE: {
       const int bc = bit(op.q[0]), bt = bit(op.q[1]);
       const uint64_t b1 = (uint64_t(op.cs) << bc) | (uint64_t(1) << bt);
-      hipLaunchKernelGGL((k_cphase<C, NP>), dim3(blocks_for(N / 4, per)), dim3(TPB), 0, st, psi,
+      QSV_LAUNCH_BLOCKS((k_cphase<C, NP>), blocks_for(N / 4, per), TPB, 0, st, psi,
                          N / 4, std::min(bc, bt), std::max(bc, bt), b1, to_c<C>(op.m[0]));
       break;
     }
     case K_SWAP: {
       const int b0 = bit(op.q[0]), b1 = bit(op.q[1]);
       const int scale = op.m[0] != cd(1.0, 0.0);
-      hipLaunchKernelGGL((k_swap<C, NP>), dim3(blocks_for(N / 4, per)), dim3(TPB), 0, st, psi,
+      QSV_LAUNCH_BLOCKS((k_swap<C, NP>), blocks_for(N / 4, per), TPB, 0, st, psi,
                          N / 4, std::min(b0, b1), std::max(b0, b1), to_c<C>(op.m[0]), scale);
       break;
     }
     case K_DENSE2: {
       M4<C> m;
       for (int i = 0; i < 16; ++i) m.a[i] = to_c<C>(op.m[i]);
-      hipLaunchKernelGGL((k_dense2<C, 1>), dim3(blocks_for(N / 4, TPB)), dim3(TPB), 0, st, psi,
+      QSV_LAUNCH_BLOCKS((k_dense2<C, 1>), blocks_for(N / 4, TPB), TPB, 0, st, psi,
                          N / 4, bit(op.q[0]), bit(op.q[1]), m);
       break;
     }
     case K_DIAG2:
-      hipLaunchKernelGGL((k_diag2<C, NP>), dim3(blocks_for(N, per)), dim3(TPB), 0, st, psi, N,
+      QSV_LAUNCH_BLOCKS((k_diag2<C, NP>), blocks_for(N, per), TPB, 0, st, psi, N,
                          bit(op.q[0]), bit(op.q[1]), to_c<C>(op.m[0]), to_c<C>(op.m[1]),
                          to_c<C>(op.m[2]), to_c<C>(op.m[3]));
       break;
@@ -373,7 +373,7 @@ template <typename R>
 int launch_set_basis(void* p, int n, uint64_t idx, hipStream_t st) {
   using C = typename CT<R>::type;
   const uint64_t N = uint64_t(1) << n;
-  hipLaunchKernelGGL((k_set_basis<C>), dim3(blocks_for(N, TPB)), dim3(TPB), 0, st,
+  QSV_LAUNCH_BLOCKS((k_set_basis<C>), blocks_for(N, TPB), TPB, 0, st,
                      static_cast<C*>(p), N, idx);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;

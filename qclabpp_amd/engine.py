@@ -67,7 +67,7 @@ def set_basis(psi, index: int = 0, stream=None) -> None:
                                   stream_handle(stream)))
 
 
-def apply_gate(psi, gate, op: str = "N", stream=None) -> None:
+def apply_gate(psi, gate, op: str | None = None, stream=None) -> None:
     n, dt = _check_state(psi)
     rec = gate.record(op)
     check(lib().qsv_dev_apply_gate(C.c_void_p(psi.data_ptr()), n, dt, C.byref(rec),

@@ -49,6 +49,7 @@ class Gate:
     ctrl_state: int = 1
     param: int = -1                  # index into a parameter vector
     mat: np.ndarray | None = None    # U1/CU: 2x2, U2: 4x4 (row/col as math)
+    op: str = "N"                    # 'N', 'T' or 'C': the gate applies op(matrix)
 
     @property
     def name(self) -> str:
@@ -58,7 +59,9 @@ class Gate:
     def nb_qubits(self) -> int:
         return 1 if self.kind in ONE_QUBIT else 2
 
-    def record(self, op: str = "N", offset: int = 0) -> QsvGate:
+    def record(self, op: str | None = None, offset: int = 0) -> QsvGate:
+        """The engine record; `op` overrides the gate's own op."""
+        op = self.op if op is None else op
         g = QsvGate()
         g.kind = self.kind
         g.q[0] = self.qubits[0] + offset
@@ -82,7 +85,19 @@ class Gate:
         return replace(self, theta=float(theta), cs=None)
 
 
-def records(gates: Iterable[Gate], op: str = "N", offset: int = 0):
+def inverse(gates: Iterable[Gate]) -> list:
+    """The inverse circuit: the gates in reverse order, each applied as its
+    conjugate transpose (QCircuit::apply with Op::ConjTrans,
+    include/qclab/QCircuit.hpp:137-142).  Gates must carry op 'N'."""
+    out = []
+    for g in reversed(list(gates)):
+        if g.op != "N":
+            raise ValueError("inverse() takes gates applied with op 'N'")
+        out.append(replace(g, op="C"))
+    return out
+
+
+def records(gates: Iterable[Gate], op: str | None = None, offset: int = 0):
     gl = list(gates)
     arr = (QsvGate * max(1, len(gl)))()
     for i, g in enumerate(gl):
