@@ -46,12 +46,13 @@ template <> struct MF<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-__device__ __forceinline__ uint32_t pdep32(uint32_t x, uint32_t mask) {
+__device__ __forceinline__ uint32_t pext32(uint32_t x, uint32_t mask) {
   uint32_t r = 0;
+  int k = 0;
   for (int b = 0; b < 32 && mask; ++b) {
     const uint32_t low = mask & (0u - mask);
-    if (x & 1u) r |= low;
-    x >>= 1;
+    if (x & low) r |= 1u << k;
+    ++k;
     mask ^= low;
   }
   return r;
@@ -77,34 +78,53 @@ __device__ __forceinline__ uint64_t deposit_runs(uint64_t w, const TileMap& tm) 
   return a;
 }
 
-template <typename R, int K>
+// Orders one wavefront's LDS accesses: every lane's earlier LDS writes are
+// visible to every lane of the same wave afterwards.  Each wave of the block
+// kernel owns its own tile region, so waves never wait for each other and
+// their load, MFMA and store phases interleave freely.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// NB MFMA column tiles (16 groups each) per wave tile.
+template <typename R, int K, int NB>
 __global__ __launch_bounds__(256) void k_fused_mfma(typename CT<R>::type* __restrict__ psi,
                                                     const R* __restrict__ ulane, TileMap tm,
                                                     uint64_t ntiles) {
   using C = typename CT<R>::type;
   using Acc = typename MF<R>::acc;
   constexpr int A = 1 << K;     // amplitudes per group
-  constexpr int TE = A * 16;    // amplitudes per wave tile
+  constexpr int GW = 16 * NB;   // groups per wave tile
+  constexpr int TE = A * GW;    // amplitudes per wave tile
   constexpr int M = 2 * A;      // real rows of U_real
   constexpr int MT = M / 16;    // output row tiles
   constexpr int KS = M / 4;     // K steps
   constexpr int EPL = TE / 64;  // tile amplitudes per lane
+  // LDS tile of one wave: amplitude i of group g sits in slot GW*i + (g ^ (i & (GW-1))).
+  // Whichever address bits the block and group bits are, the 16 lanes of an
+  // MFMA operand row (16 groups, one i) and the 16 consecutive tile positions
+  // of a global access map to 16 different bank groups.
   __shared__ C tile[4][TE];
-  __shared__ int lA[A];
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (threadIdx.x < A) lA[threadIdx.x] = int(pdep32(threadIdx.x, tm.local_block_mask));
   R areg[MT][KS];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int s = 0; s < KS; ++s) areg[mt][s] = ulane[(mt * KS + s) * 64 + lane];
-  const int lg = int(pdep32(lane & 15, tm.local_group_mask));
+  const int g = lane & 15;
   uint64_t eoff[EPL];
+  int slot[EPL];
   const uint64_t loff = tile_offset(uint32_t(lane), tm);
 #pragma unroll
-  for (int e = 0; e < EPL; ++e) eoff[e] = loff | tile_offset(uint32_t(e) << 6, tm);
-  __syncthreads();
+  for (int e = 0; e < EPL; ++e) {
+    const uint32_t t = uint32_t(lane + 64 * e);  // natural tile position
+    eoff[e] = loff | tile_offset(uint32_t(e) << 6, tm);
+    const int i = int(pext32(t, tm.local_block_mask)), gg = int(pext32(t, tm.local_group_mask));
+    slot[e] = GW * i + (gg ^ (i & (GW - 1)));
+  }
 
   for (uint64_t w0 = uint64_t(blockIdx.x) * 4; w0 < ntiles; w0 += uint64_t(gridDim.x) * 4) {
     const uint64_t w = w0 + wave;
@@ -115,35 +135,43 @@ __global__ __launch_bounds__(256) void k_fused_mfma(typename CT<R>::type* __rest
 #pragma unroll
       for (int e = 0; e < EPL; ++e) v[e] = psi[base + eoff[e]];
 #pragma unroll
-      for (int e = 0; e < EPL; ++e) tile[wave][lane + 64 * e] = v[e];
+      for (int e = 0; e < EPL; ++e) tile[wave][slot[e]] = v[e];
     }
-    __syncthreads();
-    R b[KS];
+    wave_sync();
+    Acc acc[NB][MT];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const C v = tile[wave][lA[2 * s + (lane >> 5)] + lg];
-      b[s] = (lane & 16) ? v.y : v.x;
+    for (int c = 0; c < NB; ++c) {
+      const int gc = 16 * c + g;
+      R b[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int i = 2 * s + (lane >> 5);
+        const C v = tile[wave][GW * i + (gc ^ (i & (GW - 1)))];
+        b[s] = (lane & 16) ? v.y : v.x;
+      }
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) acc[c][mt] = Acc{0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[c][mt] = MF<R>::mma(areg[mt][s], b[s], acc[c][mt]);
     }
-    Acc acc[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) acc[mt] = Acc{0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < KS; ++s)
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) acc[mt] = MF<R>::mma(areg[mt][s], b[s], acc[mt]);
-    __syncthreads();
+    wave_sync();
     R* tre = reinterpret_cast<R*>(&tile[wave][0]);
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int c = 0; c < NB; ++c)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * mt + MF<R>::row(lane, r);  // real row: 2*amp + (re|im)
-        tre[2 * (lA[row >> 1] + lg) + (row & 1)] = acc[mt][r];
-      }
-    __syncthreads();
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * mt + MF<R>::row(lane, r);  // real row: 2*amp + (re|im)
+          const int i = row >> 1, gc = 16 * c + g;
+          tre[2 * (GW * i + (gc ^ (i & (GW - 1)))) + (row & 1)] = acc[c][mt][r];
+        }
+    wave_sync();
     if (act) {
 #pragma unroll
-      for (int e = 0; e < EPL; ++e) psi[base + eoff[e]] = tile[wave][lane + 64 * e];
+      for (int e = 0; e < EPL; ++e) st_stream(&psi[base + eoff[e]], tile[wave][slot[e]]);
     }
   }
 }
@@ -177,7 +205,7 @@ __global__ __launch_bounds__(64) void k_densek_simple(typename CT<R>::type* __re
     uint64_t a = base;
     for (int p = 0; p < k; ++p)
       if (r >> p & 1) a |= uint64_t(1) << bb.b[p];
-    psi[a] = acc;
+    st_stream(&psi[a], acc);
   }
   (void)bits_lo;
 }
@@ -204,31 +232,35 @@ __global__ __launch_bounds__(256) void k_diagk(typename CT<R>::type* __restrict_
     if (e < nel) {
       int idx = 0;
       for (int p = 0; p < k; ++p) idx |= int((e >> bb.b[p]) & 1) << p;
-      psi[e] = cmul(tab[idx], x[j]);
+      st_stream(&psi[e], cmul(tab[idx], x[j]));
     }
   }
 }
+
+// Column tiles per wave tile: two for k <= 4 (twice the bytes per wave
+// iteration), one for k = 5 (its A operand already fills the registers).
+template <int K> constexpr int kColTiles = K <= 4 ? 2 : 1;
 
 template <int K, typename R>
 void launch_mfma(void* psi, const void* ulane, const TileMap& tm, uint64_t ntiles, hipStream_t st) {
   using C = typename CT<R>::type;
   const uint64_t want = (ntiles + 3) / 4;
   const unsigned grid = unsigned(want < 4096 ? want : 4096);
-  hipLaunchKernelGGL((k_fused_mfma<R, K>), dim3(grid), dim3(256), 0, st, static_cast<C*>(psi),
-                     static_cast<const R*>(ulane), tm, ntiles);
+  hipLaunchKernelGGL((k_fused_mfma<R, K, kColTiles<K>>), dim3(grid), dim3(256), 0, st,
+                     static_cast<C*>(psi), static_cast<const R*>(ulane), tm, ntiles);
 }
 
 }  // namespace
 
-TileMap make_tile_map(const std::vector<int>& qubits, int n) {
+TileMap make_tile_map(const std::vector<int>& qubits, int n, int group_bits) {
   TileMap tm{};
   const int k = int(qubits.size());
   uint64_t bmask = 0;
   for (int q : qubits) bmask |= uint64_t(1) << (n - 1 - q);
-  // group bits: the four lowest bits outside the block
+  // group bits: the lowest bits outside the block
   uint64_t gmask = 0;
   int got = 0;
-  for (int b = 0; b < n && got < 4; ++b)
+  for (int b = 0; b < n && got < group_bits; ++b)
     if (!(bmask >> b & 1)) {
       gmask |= uint64_t(1) << b;
       ++got;
@@ -273,8 +305,9 @@ int launch_fused(const KOp& op, const DeviceBlock& db, void* psi, int n, hipStre
                       static_cast<const C*>(db.table), k, bb);
   } else if (op.kind == K_DENSEK) {
     if (k >= 3 && n >= k + 6) {
-      const TileMap tm = make_tile_map(std::vector<int>(op.q, op.q + k), n);
-      const uint64_t ntiles = N >> (k + 4);
+      const int gbits = 4 + (k <= 4 ? 1 : 0);  // log2(16 * column tiles)
+      const TileMap tm = make_tile_map(std::vector<int>(op.q, op.q + k), n, gbits);
+      const uint64_t ntiles = N >> (k + gbits);
       switch (k) {
         case 3: launch_mfma<3, R>(psi, db.ulane, tm, ntiles, st); break;
         case 4: launch_mfma<4, R>(psi, db.ulane, tm, ntiles, st); break;

@@ -27,7 +27,7 @@ struct TileMap {
   uint8_t rlen[16];
 };
 
-TileMap make_tile_map(const std::vector<int>& qubits, int n);
+TileMap make_tile_map(const std::vector<int>& qubits, int n, int group_bits);
 
 // Device copies of one block's coefficients.
 struct DeviceBlock {

@@ -60,8 +60,8 @@ __global__ __launch_bounds__(TPB) void k_dense1(C* __restrict__ psi, uint64_t cn
     const uint64_t k = k0 + uint64_t(j) * TPB;
     if (k < cnt) {
       const uint64_t a = ins0(k, b);
-      psi[a] = cmad(m.a01, y[j], cmul(m.a00, x[j]));
-      psi[a | s] = cmad(m.a11, y[j], cmul(m.a10, x[j]));
+      st_stream(&psi[a], cmad(m.a01, y[j], cmul(m.a00, x[j])));
+      st_stream(&psi[a | s], cmad(m.a11, y[j], cmul(m.a10, x[j])));
     }
   }
 }
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(TPB) void k_dense1_lanes(C* __restrict__ psi, uint6
     const bool hi = (e >> b) & 1;
     const C cs = hi ? m.a11 : m.a00;  // coefficient on own amplitude
     const C co = hi ? m.a10 : m.a01;  // coefficient on partner amplitude
-    if (e < nel) psi[e] = cmad(co, y, cmul(cs, x[j]));
+    if (e < nel) st_stream(&psi[e], cmad(co, y, cmul(cs, x[j])));
   }
 }
 
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(TPB) void k_diag1(C* __restrict__ psi, uint64_t nel
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t e = e0 + uint64_t(j) * TPB;
-    if (e < nel) psi[e] = cmul(((e >> b) & 1) ? d1 : d0, x[j]);
+    if (e < nel) st_stream(&psi[e], cmul(((e >> b) & 1) ? d1 : d0, x[j]));
   }
 }
 
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cn
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t k = k0 + uint64_t(j) * TPB;
-    if (k < cnt) psi[ins0(k, b) | s] = cmul(lam, x[j]);
+    if (k < cnt) st_stream(&psi[ins0(k, b) | s], cmul(lam, x[j]));
   }
 }
 
@@ -146,8 +146,8 @@ __global__ __launch_bounds__(TPB) void k_cdense1(C* __restrict__ psi, uint64_t c
     const uint64_t k = k0 + uint64_t(j) * TPB;
     if (k < cnt) {
       const uint64_t a = ins0(ins0(k, blo), bhi) | a1;
-      psi[a] = cmad(m.a01, y[j], cmul(m.a00, x[j]));
-      psi[a | s] = cmad(m.a11, y[j], cmul(m.a10, x[j]));
+      st_stream(&psi[a], cmad(m.a01, y[j], cmul(m.a00, x[j])));
+      st_stream(&psi[a | s], cmad(m.a11, y[j], cmul(m.a10, x[j])));
     }
   }
 }
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(TPB) void k_cdense1_lanes(C* __restrict__ psi, uint
     const bool hi = (e >> bt) & 1;
     const C cself = hi ? m.a11 : m.a00;
     const C cother = hi ? m.a10 : m.a01;
-    if (e < nel && int((e >> bc) & 1) == cs) psi[e] = cmad(cother, y, cmul(cself, x[j]));
+    if (e < nel && int((e >> bc) & 1) == cs) st_stream(&psi[e], cmad(cother, y, cmul(cself, x[j])));
   }
 }
 
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(TPB) void k_cphase(C* __restrict__ psi, uint64_t cn
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t k = k0 + uint64_t(j) * TPB;
-    if (k < cnt) psi[ins0(ins0(k, blo), bhi) | b1] = cmul(lam, x[j]);
+    if (k < cnt) st_stream(&psi[ins0(ins0(k, blo), bhi) | b1], cmul(lam, x[j]));
   }
 }
 
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(TPB) void k_swap(C* __restrict__ psi, uint64_t cnt,
     const uint64_t k = k0 + uint64_t(j) * TPB;
     if (k < cnt) {
       const uint64_t i = ins0(ins0(k, blo), bhi);
-      psi[i | sh] = scale ? cmul(lam, y[j]) : y[j];
-      psi[i | sl] = scale ? cmul(lam, x[j]) : x[j];
+      st_stream(&psi[i | sh], scale ? cmul(lam, y[j]) : y[j]);
+      st_stream(&psi[i | sl], scale ? cmul(lam, x[j]) : x[j]);
     }
   }
 }
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(TPB) void k_dense2(C* __restrict__ psi, uint64_t cn
         y[r] = acc;
       }
 #pragma unroll
-      for (int p = 0; p < 4; ++p) psi[ix[p]] = y[p];
+      for (int p = 0; p < 4; ++p) st_stream(&psi[ix[p]], y[p]);
     }
   }
 }
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(TPB) void k_diag2(C* __restrict__ psi, uint64_t nel
     if (e < nel) {
       const int p = int(((e >> b0) & 1) << 1) | int((e >> b1) & 1);
       const C d = p == 0 ? d0 : p == 1 ? d1 : p == 2 ? d2 : d3;
-      psi[e] = cmul(d, x[j]);
+      st_stream(&psi[e], cmul(d, x[j]));
     }
   }
 }
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(TPB) void k_diag2(C* __restrict__ psi, uint64_t nel
 template <typename C>
 __global__ __launch_bounds__(TPB) void k_set_basis(C* __restrict__ psi, uint64_t nel, uint64_t idx, uint64_t blk0) {
   const uint64_t e = (blk0 + blockIdx.x) * TPB + threadIdx.x;
-  if (e < nel) psi[e] = mk<C>(e == idx ? 1 : 0, 0);
+  if (e < nel) st_stream(&psi[e], mk<C>(e == idx ? 1 : 0, 0));
 }
 
 template <typename C>
