@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_fused_mfma(typename CT<R>::type* __rest
     if (act) {
       C v[EPL];
 #pragma unroll
-      for (int e = 0; e < EPL; ++e) v[e] = psi[base + eoff[e]];
+      for (int e = 0; e < EPL; ++e) v[e] = ld_stream(&psi[base + eoff[e]]);
 #pragma unroll
       for (int e = 0; e < EPL; ++e) tile[wave][slot[e]] = v[e];
     }
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(64) void k_densek_simple(typename CT<R>::type* __re
     uint64_t a = base;
     for (int p = 0; p < k; ++p)
       if (r >> p & 1) a |= uint64_t(1) << bb.b[p];
-    x[r] = psi[a];
+    x[r] = ld_stream(&psi[a]);
   }
   for (int r = 0; r < D; ++r) {
     C acc = mk<C>(0, 0);
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_diagk(typename CT<R>::type* __restrict_
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t e = e0 + uint64_t(j) * 256;
-    if (e < nel) x[j] = psi[e];
+    if (e < nel) x[j] = ld_stream(&psi[e]);
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j) {

@@ -51,8 +51,8 @@ __global__ __launch_bounds__(TPB) void k_dense1(C* __restrict__ psi, uint64_t cn
     const uint64_t k = k0 + uint64_t(j) * TPB;
     if (k < cnt) {
       const uint64_t a = ins0(k, b);
-      x[j] = psi[a];
-      y[j] = psi[a | s];
+      x[j] = ld_stream(&psi[a]);
+      y[j] = ld_stream(&psi[a | s]);
     }
   }
 #pragma unroll
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(TPB) void k_dense1_lanes(C* __restrict__ psi, uint6
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t e = e0 + uint64_t(j) * TPB;
-    x[j] = e < nel ? psi[e] : mk<C>(0, 0);
+    x[j] = e < nel ? ld_stream(&psi[e]) : mk<C>(0, 0);
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(TPB) void k_diag1(C* __restrict__ psi, uint64_t nel
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t e = e0 + uint64_t(j) * TPB;
-    if (e < nel) x[j] = psi[e];
+    if (e < nel) x[j] = ld_stream(&psi[e]);
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cn
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t k = k0 + uint64_t(j) * TPB;
-    if (k < cnt) x[j] = psi[ins0(k, b) | s];
+    if (k < cnt) x[j] = ld_stream(&psi[ins0(k, b) | s]);
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -137,8 +137,8 @@ __global__ __launch_bounds__(TPB) void k_cdense1(C* __restrict__ psi, uint64_t c
     const uint64_t k = k0 + uint64_t(j) * TPB;
     if (k < cnt) {
       const uint64_t a = ins0(ins0(k, blo), bhi) | a1;
-      x[j] = psi[a];
-      y[j] = psi[a | s];
+      x[j] = ld_stream(&psi[a]);
+      y[j] = ld_stream(&psi[a | s]);
     }
   }
 #pragma unroll
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(TPB) void k_cdense1_lanes(C* __restrict__ psi, uint
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t e = e0 + uint64_t(j) * TPB;
-    x[j] = e < nel ? psi[e] : mk<C>(0, 0);
+    x[j] = e < nel ? ld_stream(&psi[e]) : mk<C>(0, 0);
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(TPB) void k_cphase(C* __restrict__ psi, uint64_t cn
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t k = k0 + uint64_t(j) * TPB;
-    if (k < cnt) x[j] = psi[ins0(ins0(k, blo), bhi) | b1];
+    if (k < cnt) x[j] = ld_stream(&psi[ins0(ins0(k, blo), bhi) | b1]);
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -206,8 +206,8 @@ __global__ __launch_bounds__(TPB) void k_swap(C* __restrict__ psi, uint64_t cnt,
     const uint64_t k = k0 + uint64_t(j) * TPB;
     if (k < cnt) {
       const uint64_t i = ins0(ins0(k, blo), bhi);
-      x[j] = psi[i | sh];
-      y[j] = psi[i | sl];
+      x[j] = ld_stream(&psi[i | sh]);
+      y[j] = ld_stream(&psi[i | sl]);
     }
   }
 #pragma unroll
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(TPB) void k_dense2(C* __restrict__ psi, uint64_t cn
       const uint64_t ix[4] = {i, i | s1, i | s0, i | s0 | s1};
       C x[4], y[4];
 #pragma unroll
-      for (int p = 0; p < 4; ++p) x[p] = psi[ix[p]];
+      for (int p = 0; p < 4; ++p) x[p] = ld_stream(&psi[ix[p]]);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         C acc = cmul(m.a[4 * r], x[0]);
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(TPB) void k_diag2(C* __restrict__ psi, uint64_t nel
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     const uint64_t e = e0 + uint64_t(j) * TPB;
-    if (e < nel) x[j] = psi[e];
+    if (e < nel) x[j] = ld_stream(&psi[e]);
   }
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
