@@ -221,6 +221,89 @@ __global__ __launch_bounds__(TPB) void k_swap(C* __restrict__ psi, uint64_t cnt,
   }
 }
 
+// ---- SWAP / iSWAP whose lower bit is below the lane count ------------------
+// Each thread owns the pair (e, e|sh) for one e with bit bhi clear, loaded and
+// stored whole, so every 128-byte line is read and written in full; the
+// |01>,|10> exchange crosses to lane ^ 2^blo.  Thread k < cnt = N/2.
+template <typename C>
+__global__ __launch_bounds__(TPB) void k_swap_lanes(C* __restrict__ psi, uint64_t cnt, int blo,
+                                                    int bhi, C lam, int scale, uint64_t blk0) {
+  const uint64_t k = (blk0 + blockIdx.x) * TPB + threadIdx.x;
+  const uint64_t sh = uint64_t(1) << bhi;
+  const uint64_t e = ins0(k < cnt ? k : 0, bhi);
+  C a = ld_stream(&psi[e]), b = ld_stream(&psi[e | sh]);
+  const bool lo = (e >> blo) & 1;
+  // lo = 0 holds (|00>, |10>) and hands |10> over; lo = 1 holds (|01>, |11>) and hands |01>
+  const C v = shfl_xor_c(lo ? a : b, 1 << blo);
+  const C w = scale ? cmul(lam, v) : v;
+  if (lo) a = w;
+  else b = w;
+  if (k < cnt) {
+    st_stream(&psi[e], a);
+    st_stream(&psi[e | sh], b);
+  }
+}
+
+// ---- controlled 2x2, target bit below the lane count, control bit above it -----
+// Sweeps only the half with control bit == cs (thread k < N/2 owns element
+// ins0(k, bc) | cs<<bc); the target partner is lane ^ 2^bt.
+template <typename C>
+__global__ __launch_bounds__(TPB) void k_cdense1_lanes_half(C* __restrict__ psi, uint64_t cnt,
+                                                            int bt, int bc, int cs, M2<C> m,
+                                                            uint64_t blk0) {
+  const uint64_t k = (blk0 + blockIdx.x) * TPB + threadIdx.x;
+  const uint64_t e = ins0(k < cnt ? k : 0, bc) | (uint64_t(cs) << bc);
+  const C x = ld_stream(&psi[e]);
+  const C y = shfl_xor_c(x, 1 << bt);
+  const bool hi = (e >> bt) & 1;
+  const C cself = hi ? m.a11 : m.a00;
+  const C cother = hi ? m.a10 : m.a01;
+  if (k < cnt) st_stream(&psi[e], cmad(cother, y, cmul(cself, x)));
+}
+
+// ---- phase on a quarter whose lower bit lies inside a 128-byte line ----------
+// Sweeps the half with bit bh == vh in whole lines and scales the amplitudes
+// whose bit bl == vl (thread k < N/2 owns element ins0(k, bh) | vh<<bh).
+template <typename C>
+__global__ __launch_bounds__(TPB) void k_cphase_half(C* __restrict__ psi, uint64_t cnt, int bh,
+                                                     int vh, int bl, int vl, C lam, uint64_t blk0) {
+  const uint64_t k = (blk0 + blockIdx.x) * TPB + threadIdx.x;
+  if (k >= cnt) return;
+  const uint64_t e = ins0(k, bh) | (uint64_t(vh) << bh);
+  const C x = ld_stream(&psi[e]);
+  st_stream(&psi[e], int((e >> bl) & 1) == vl ? cmul(lam, x) : x);
+}
+
+// ---- controlled 2x2 whose control bit is inside a 128-byte line --------------
+// Sweeps every target pair (like k_dense1) and applies the matrix where the
+// control bit equals cs, identity elsewhere, so whole lines are written.
+template <typename C, int NP>
+__global__ __launch_bounds__(TPB) void k_cdense1_full(C* __restrict__ psi, uint64_t cnt, int bt,
+                                                      int bc, int cs, M2<C> m, uint64_t blk0) {
+  const uint64_t s = uint64_t(1) << bt;
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP], y[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t a = ins0(k, bt);
+      x[j] = ld_stream(&psi[a]);
+      y[j] = ld_stream(&psi[a | s]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    if (k < cnt) {
+      const uint64_t a = ins0(k, bt);
+      const bool on = int((a >> bc) & 1) == cs;
+      st_stream(&psi[a], on ? cmad(m.a01, y[j], cmul(m.a00, x[j])) : x[j]);
+      st_stream(&psi[a | s], on ? cmad(m.a11, y[j], cmul(m.a10, x[j])) : y[j]);
+    }
+  }
+}
+
 // ---- apply4 (dense): 4x4 on quads ----------------------------------------------
 // b0 = bit of the lower-numbered qubit (more significant), b1 of the other;
 // quad position p = 2*bit(b0) + bit(b1).
@@ -296,8 +379,13 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
   using C = typename CT<R>::type;
   C* psi = static_cast<C*>(p);
   const uint64_t N = uint64_t(1) << n;
-  constexpr int NP = 4;
+  // items per thread, measured on gfx950 at n=30: one for the full and half
+  // sweeps (more waves in flight), two for the controlled quarter-pair sweep
+  constexpr int NP = 1, NPC = 2;
   const int per = TPB * NP;
+  // address bits inside one 128-byte line: a sweep that touches only some
+  // amplitudes of a line is replaced by one that reads and writes whole lines
+  const int line_bits = sizeof(C) == 16 ? 3 : 4;
   auto bit = [n](int q) { return n - 1 - q; };
   switch (op.kind) {
     case K_NOP:
@@ -319,35 +407,65 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
                          bit(op.q[0]), to_c<C>(op.m[0]), to_c<C>(op.m[1]));
       break;
     case K_PHASE1:
-      QSV_LAUNCH_BLOCKS((k_phase1<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi,
-                         N / 2, bit(op.q[0]), to_c<C>(op.m[0]));
+      if (bit(op.q[0]) < line_bits) {  // the |1> half interleaves within lines: sweep all
+        QSV_LAUNCH_BLOCKS((k_diag1<C, NP>), blocks_for(N, per), TPB, 0, st, psi, N, bit(op.q[0]),
+                          mk<C>(1, 0), to_c<C>(op.m[0]));
+      } else {
+        QSV_LAUNCH_BLOCKS((k_phase1<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi,
+                          N / 2, bit(op.q[0]), to_c<C>(op.m[0]));
+      }
       break;
     case K_CDENSE1: {
       const M2<C> m{to_c<C>(op.m[0]), to_c<C>(op.m[1]), to_c<C>(op.m[2]), to_c<C>(op.m[3])};
       const int bc = bit(op.q[0]), bt = bit(op.q[1]);
-      if (bt < tu.lane_bits && n >= 6) {
+      if (bt < tu.lane_bits && bc > bt && n >= 7) {
+        QSV_LAUNCH_BLOCKS((k_cdense1_lanes_half<C>), blocks_for(N / 2, TPB), TPB, 0, st, psi,
+                          N / 2, bt, bc, op.cs, m);
+      } else if (bt < tu.lane_bits && n >= 6) {
         QSV_LAUNCH_BLOCKS((k_cdense1_lanes<C, NP>), blocks_for(N, per), TPB, 0, st,
-                           psi, N, bt, bc, op.cs, m);
+                          psi, N, bt, bc, op.cs, m);
+      } else if (bc < line_bits) {
+        QSV_LAUNCH_BLOCKS((k_cdense1_full<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi, N / 2,
+                          bt, bc, op.cs, m);
       } else {
         const uint64_t a1 = uint64_t(op.cs) << bc;
-        QSV_LAUNCH_BLOCKS((k_cdense1<C, NP>), blocks_for(N / 4, per), TPB, 0, st,
-                           psi, N / 4, std::min(bc, bt), std::max(bc, bt), a1,
-                           uint64_t(1) << bt, m);
+        QSV_LAUNCH_BLOCKS((k_cdense1<C, NPC>), blocks_for(N / 4, TPB * NPC), TPB, 0, st,
+                          psi, N / 4, std::min(bc, bt), std::max(bc, bt), a1,
+                          uint64_t(1) << bt, m);
       }
       break;
     }
     case K_CPHASE: {
       const int bc = bit(op.q[0]), bt = bit(op.q[1]);
-      const uint64_t b1 = (uint64_t(op.cs) << bc) | (uint64_t(1) << bt);
-      QSV_LAUNCH_BLOCKS((k_cphase<C, NP>), blocks_for(N / 4, per), TPB, 0, st, psi,
-                         N / 4, std::min(bc, bt), std::max(bc, bt), b1, to_c<C>(op.m[0]));
+      if (std::min(bc, bt) < line_bits && std::max(bc, bt) >= line_bits) {
+        // the quarter's lines are the half with the upper bit set as the pattern asks
+        const bool ch = bc > bt;
+        QSV_LAUNCH_BLOCKS((k_cphase_half<C>), blocks_for(N / 2, TPB), TPB, 0, st, psi, N / 2,
+                          ch ? bc : bt, ch ? op.cs : 1, ch ? bt : bc, ch ? 1 : op.cs,
+                          to_c<C>(op.m[0]));
+      } else if (std::min(bc, bt) < line_bits) {  // both bits inside a line: sweep all
+        const C one = mk<C>(1, 0), lam = to_c<C>(op.m[0]);
+        // k_diag2 index p = 2*bit(bc) + bit(bt): the phase sits at (cs, 1)
+        QSV_LAUNCH_BLOCKS((k_diag2<C, NP>), blocks_for(N, per), TPB, 0, st, psi, N, bc, bt, one,
+                          op.cs == 0 ? lam : one, one, op.cs == 1 ? lam : one);
+      } else {
+        const uint64_t b1 = (uint64_t(op.cs) << bc) | (uint64_t(1) << bt);
+        QSV_LAUNCH_BLOCKS((k_cphase<C, NP>), blocks_for(N / 4, per), TPB, 0, st, psi,
+                          N / 4, std::min(bc, bt), std::max(bc, bt), b1, to_c<C>(op.m[0]));
+      }
       break;
     }
     case K_SWAP: {
       const int b0 = bit(op.q[0]), b1 = bit(op.q[1]);
+      const int blo = std::min(b0, b1), bhi = std::max(b0, b1);
       const int scale = op.m[0] != cd(1.0, 0.0);
-      QSV_LAUNCH_BLOCKS((k_swap<C, NP>), blocks_for(N / 4, per), TPB, 0, st, psi,
-                         N / 4, std::min(b0, b1), std::max(b0, b1), to_c<C>(op.m[0]), scale);
+      if (blo < line_bits && n >= 7) {  // the pairs interleave within lines: whole-line pairs
+        QSV_LAUNCH_BLOCKS((k_swap_lanes<C>), blocks_for(N / 2, TPB), TPB, 0, st, psi, N / 2, blo,
+                          bhi, to_c<C>(op.m[0]), scale);
+      } else {
+        QSV_LAUNCH_BLOCKS((k_swap<C, NP>), blocks_for(N / 4, per), TPB, 0, st, psi,
+                          N / 4, blo, bhi, to_c<C>(op.m[0]), scale);
+      }
       break;
     }
     case K_DENSE2: {

@@ -53,11 +53,14 @@ def test_every_gate_small(n, dtype):
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_every_qubit_large(dtype):
     """n=14: all 14 qubits as target (both lane and pair kernels), a spread
-    of control/target pairs, multi-block grids."""
+    of control/target pairs, multi-block grids.  Pairs with a bit inside one
+    128-byte line (bits 0-2 for complex128, 0-3 for complex64) take the
+    whole-line variants of the half- and quarter-touch sweeps."""
     n = 14
     print(f"plain gates n={n} {dtype.__name__}", flush=True)
     psi = random_state(n, 99, dtype)
-    pairs = [(0, 13), (13, 0), (12, 13), (13, 12), (3, 9), (9, 3), (0, 1), (7, 8), (2, 12)]
+    pairs = [(0, 13), (13, 0), (12, 13), (13, 12), (3, 9), (9, 3), (0, 1), (7, 8), (2, 12),
+             (10, 2), (2, 10), (11, 4), (4, 11)]
     cases = [(g, op) for q in range(n) for g in one_qubit_gates(q) for op in "NC"]
     cases += [(g, op) for a, b in pairs for g in two_qubit_gates(a, b) for op in "NC"]
     for gate, op in cases:
