@@ -170,8 +170,13 @@ int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates,
  * fused: with fuse_k in 1..5 runs of gates collapse into dense 2^k x 2^k
  * blocks (applied with MFMA for k >= 3) and runs of diagonal gates into
  * phase tables over up to diag_k qubits (diag_k <= 10; 0 picks 8).
- * fuse_k == 0 keeps one plain launch per gate.  Coefficients live on the
- * device for the program's lifetime; running it allocates nothing.        */
+ * fuse_k == 0 keeps one plain launch per gate.
+ * fuse_k == QSV_TILED runs the circuit as tiled passes: each launch streams
+ * the state once through LDS tiles of 2^12 amplitudes and applies every gate
+ * of the pass there, one by one in registers (states of fewer than 12
+ * qubits use fuse_k = 5 instead).  Coefficients live on the device for the
+ * program's lifetime; running it allocates nothing.                       */
+#define QSV_TILED (-1)
 typedef struct qsv_program qsv_program;
 
 typedef struct qsv_program_info {
@@ -182,6 +187,7 @@ typedef struct qsv_program_info {
   int64_t ndense[6];   /* dense fused blocks by size k (k >= 3)         */
   double  alg_bytes;   /* algorithmic HBM bytes of one run              */
   double  flops;       /* useful flops of the dense fused blocks        */
+  int64_t ntiled_phases; /* tiled programs: register phases of one run  */
 } qsv_program_info;
 
 int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,
@@ -289,7 +295,7 @@ int qsv_dev_max_abs_diff(const void* a, const void* b, uint64_t count, int dtype
 int qsv_host_apply_gates(void* host, int n, int dtype, const qsv_gate* gates,
                          size_t ngates);
 /* Same, running the gate list as one compiled program with fusion width
- * fuse_k (0 = plain launches).                                            */
+ * fuse_k (0 = plain launches, 1..5 = fused, QSV_TILED = tiled passes).   */
 int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_t ngates,
                       int fuse_k);
 /* Device-resident variant: compile, run on `psi`, synchronise `stream`.   */

@@ -104,11 +104,16 @@ def _ptr(arr: np.ndarray):
     return arr.ctypes.data_as(C.c_void_p)
 
 
-def apply(n: int, gates, psi: np.ndarray, op: str = "N") -> np.ndarray:
-    """Oracle QCircuit::apply(op) of `gates` on a copy of psi."""
+def apply(n: int, gates, psi: np.ndarray, op: str | None = None) -> np.ndarray:
+    """Oracle run of `gates` on a copy of psi.  op=None applies each gate with
+    its own op, in order; op='N'/'T'/'C' is QCircuit::apply(op): every gate
+    with that op, in reverse order for 'T'/'C' (QCircuit.hpp:130-143)."""
     out = np.ascontiguousarray(psi).copy()
     arr, cnt = records(gates)
-    st = oracle_lib().oracle_apply_gates_op(n, _dtype_code(out), arr, cnt, ord(op), _ptr(out))
+    if op is None:
+        st = oracle_lib().oracle_apply_gates(n, _dtype_code(out), arr, cnt, _ptr(out))
+    else:
+        st = oracle_lib().oracle_apply_gates_op(n, _dtype_code(out), arr, cnt, ord(op), _ptr(out))
     if st:
         raise RuntimeError(f"oracle status {st}")
     return out

@@ -14,6 +14,7 @@ LIB_PATH = PKG / "libqclabpp_amd.so"
 HEADER = PKG.parent / "include" / "qsv.h"
 
 QSV_OK, QSV_EINVAL, QSV_EQUBIT, QSV_ENOMEM, QSV_EHIP, QSV_EDTYPE, QSV_ENODEV, QSV_EUNSUP = range(8)
+QSV_TILED = -1  # fuse_k value selecting tiled passes (include/qsv.h)
 QSV_C64, QSV_C128 = 1, 2
 
 
@@ -78,6 +79,7 @@ class QsvProgramInfo(C.Structure):
         ("ndense", C.c_int64 * 6),
         ("alg_bytes", C.c_double),
         ("flops", C.c_double),
+        ("ntiled_phases", C.c_int64),
     ]
 
 

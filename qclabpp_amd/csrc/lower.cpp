@@ -275,16 +275,26 @@ int lower_gate(const qsv_gate& g, int n, KOp& o) {
   return QSV_OK;
 }
 
+// Minimal HBM traffic of one launch: every 128-byte line that holds a touched
+// amplitude is read and written once.  A sweep that touches the amplitudes
+// with some bits fixed touches 2^-f of the lines, f = the fixed bits at or
+// above the line's own address bits (3 for complex128, 4 for complex64).
 double algorithmic_bytes(const KOp& op, int n, int eb) {
   const double S = std::ldexp(1.0, n) * eb;
+  const int line_bits = eb == 16 ? 3 : 4;
+  auto above = [&](int q) { return (n - 1 - q) >= line_bits ? 1 : 0; };
   switch (op.kind) {
     case K_DENSE1: case K_DIAG1: case K_DENSE2: case K_DIAG2:
     case K_DENSEK: case K_DIAGK:
       return 2 * S;
-    case K_PHASE1: case K_CDENSE1: case K_SWAP:
-      return S;
-    case K_CPHASE:
-      return S / 2;
+    case K_PHASE1:                       // the |1> half of the target
+      return 2 * S / (1 << above(op.q[0]));
+    case K_CDENSE1:                      // the control == cs half
+      return 2 * S / (1 << above(op.q[0]));
+    case K_CPHASE:                       // control == cs and target == 1
+      return 2 * S / (1 << (above(op.q[0]) + above(op.q[1])));
+    case K_SWAP:                         // |01> and |10>: half the lines only if both bits are above
+      return above(op.q[0]) && above(op.q[1]) ? S : 2 * S;
     default:
       return 0;
   }

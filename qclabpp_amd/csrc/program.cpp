@@ -71,6 +71,20 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
   ngates_in = ngates;
   std::vector<KOp> lowered(ngates);
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, lowered[i]));
+  if (fuse_k == QSV_TILED && n < kQTileBits) fuse_k = kMaxFuse;  // one tile would be the whole state
+  if (fuse_k == QSV_TILED) {
+    tiled = true;
+    QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt));
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    qt_pass_off = 0;
+    qt_phase_off = up(qt.passes.size() * sizeof(QPass));
+    qt_op_off = qt_phase_off + up(qt.phases.size() * sizeof(QPhase));
+    image.assign(qt_op_off + up(qt.ops.size() * sizeof(QOp)), 0);
+    std::memcpy(image.data() + qt_pass_off, qt.passes.data(), qt.passes.size() * sizeof(QPass));
+    std::memcpy(image.data() + qt_phase_off, qt.phases.data(), qt.phases.size() * sizeof(QPhase));
+    std::memcpy(image.data() + qt_op_off, qt.ops.data(), qt.ops.size() * sizeof(QOp));
+    return QSV_OK;
+  }
   if (fuse_k >= 1) {
     FusionOptions fo;
     fo.max_k = fuse_k;
@@ -136,6 +150,20 @@ int Program::run(void* psi, hipStream_t st) const {
   QSV_TRY(upload());
   PlainTuning tu;
   const bool dbl = dtype == QSV_C128;
+  if (tiled) {
+    const unsigned char* base = static_cast<const unsigned char*>(dbuf);
+    const QPass* passes = reinterpret_cast<const QPass*>(base + qt_pass_off);
+    QTileLaunch a;
+    a.psi = psi;
+    a.phases = reinterpret_cast<const QPhase*>(base + qt_phase_off);
+    a.ops = reinterpret_cast<const QOp*>(base + qt_op_off);
+    for (size_t i = 0; i < qt.passes.size(); ++i) {
+      a.pass = passes + i;
+      a.ntiles = qt.passes[i].ntiles;
+      QSV_TRY(dbl ? launch_qtile<double>(a, st) : launch_qtile<float>(a, st));
+    }
+    return QSV_OK;
+  }
   for (const KOp& op : ops) {
     if (op.kind == K_DENSEK || op.kind == K_DIAGK) {
       const DeviceBlock& db = dblocks[op.blk];
@@ -150,6 +178,13 @@ int Program::run(void* psi, hipStream_t st) const {
 void Program::info(qsv_program_info* out) const {
   std::memset(out, 0, sizeof(*out));
   out->ngates = int64_t(ngates_in);
+  if (tiled) {
+    const int eb = dtype == QSV_C128 ? 16 : 8;
+    out->nlaunch = int64_t(qt.passes.size());
+    out->ntiled_phases = int64_t(qt.phases.size());
+    out->alg_bytes = double(qt.passes.size()) * 2.0 * std::ldexp(1.0, n) * eb;
+    return;
+  }
   out->nlaunch = int64_t(ops.size());
   const int eb = dtype == QSV_C128 ? 16 : 8;
   for (const KOp& op : ops) {

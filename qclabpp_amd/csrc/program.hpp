@@ -7,6 +7,7 @@
 #include "common.hpp"
 #include "fused.hpp"
 #include "fusion.hpp"
+#include "qtile.hpp"
 
 struct qsv_program_info;
 
@@ -22,6 +23,11 @@ struct Program {
   // device by upload() (first run).
   std::vector<unsigned char> image;
   mutable std::vector<DeviceBlock> dblocks;
+  // tiled execution (fuse_k == QSV_TILED): passes, phases and ops; their
+  // device copies sit in dbuf at the offsets below
+  bool tiled = false;
+  QProgram qt;
+  size_t qt_pass_off = 0, qt_phase_off = 0, qt_op_off = 0;
   mutable void* dbuf = nullptr;
   mutable bool uploaded = false;
   mutable std::mutex mu;

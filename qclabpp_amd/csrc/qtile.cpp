@@ -1,0 +1,351 @@
+// Host compile of the tiled execution: passes (which tile bits, which gates)
+// and phases (which register bits, which gates, in order).
+#include "qtile.hpp"
+
+#include <algorithm>
+#include <set>
+
+namespace qsv {
+
+namespace {
+
+bool is_one_qubit(const KOp& op) {
+  return op.kind == K_DENSE1 || op.kind == K_DIAG1 || op.kind == K_PHASE1;
+}
+
+// 2x2 of a 1-qubit op, row-major
+void as2x2(const KOp& o, cd* m) {
+  if (o.kind == K_DENSE1) {
+    for (int i = 0; i < 4; ++i) m[i] = o.m[i];
+  } else if (o.kind == K_DIAG1) {
+    m[0] = o.m[0]; m[1] = 0.0; m[2] = 0.0; m[3] = o.m[1];
+  } else {  // K_PHASE1
+    m[0] = 1.0; m[1] = 0.0; m[2] = 0.0; m[3] = o.m[0];
+  }
+}
+
+std::vector<int> qubits_of(const KOp& op) { return std::vector<int>(op.q, op.q + op.nq); }
+
+// Runs of 1-qubit ops on one qubit become one op (gates on other qubits in
+// between commute with them).
+std::vector<KOp> merge_one_qubit_runs(const std::vector<KOp>& in, int n) {
+  std::vector<KOp> ops;
+  std::vector<int> last(n, -1);
+  for (const KOp& op : in) {
+    if (op.kind == K_NOP) continue;
+    if (is_one_qubit(op) && last[op.q[0]] >= 0) {
+      KOp& prev = ops[last[op.q[0]]];
+      cd a[4], b[4];
+      as2x2(op, a);
+      as2x2(prev, b);
+      const bool diag = op.kind != K_DENSE1 && prev.kind != K_DENSE1;
+      const cd c0 = a[0] * b[0] + a[1] * b[2], c1 = a[0] * b[1] + a[1] * b[3];
+      const cd c2 = a[2] * b[0] + a[3] * b[2], c3 = a[2] * b[1] + a[3] * b[3];
+      if (diag) {
+        prev.kind = K_DIAG1;
+        prev.m[0] = c0;
+        prev.m[1] = c3;
+      } else {
+        prev.kind = K_DENSE1;
+        prev.m[0] = c0; prev.m[1] = c1; prev.m[2] = c2; prev.m[3] = c3;
+      }
+      continue;
+    }
+    for (int q : qubits_of(op)) last[q] = -1;
+    ops.push_back(op);
+    if (is_one_qubit(op)) last[op.q[0]] = int(ops.size()) - 1;
+  }
+  return ops;
+}
+
+// Ops of `rem` (in order) that a tile over qubit set S absorbs: those whose
+// qubits all lie in S and do not depend on an op left out.
+int count_absorbed(const std::vector<KOp>& rem, size_t limit, const std::vector<char>& inS,
+                   std::vector<char>& blocked) {
+  std::fill(blocked.begin(), blocked.end(), 0);
+  int cnt = 0;
+  const size_t m = std::min(limit, rem.size());
+  for (size_t i = 0; i < m; ++i) {
+    const KOp& op = rem[i];
+    bool ok = true;
+    for (int j = 0; j < op.nq; ++j) ok = ok && inS[op.q[j]] && !blocked[op.q[j]];
+    if (ok) ++cnt;
+    else
+      for (int j = 0; j < op.nq; ++j) blocked[op.q[j]] = 1;
+  }
+  return cnt;
+}
+
+struct Group {
+  std::vector<int> q;    // sorted qubits
+  std::vector<int> ops;  // indices into the pass's op list, ascending
+};
+
+// Fusion-style grouping of a pass's ops into blocks of at most `limit`
+// qubits, emitted in a valid order, then packed into phases.
+std::vector<Group> make_phases(const std::vector<KOp>& ops, int limit) {
+  std::vector<Group> open, emitted;
+  for (int i = 0; i < int(ops.size()); ++i) {
+    std::vector<int> Q = qubits_of(ops[i]);
+    std::sort(Q.begin(), Q.end());
+    std::vector<int> hit;
+    for (int b = 0; b < int(open.size()); ++b)
+      for (int q : Q)
+        if (std::binary_search(open[b].q.begin(), open[b].q.end(), q)) {
+          hit.push_back(b);
+          break;
+        }
+    for (;;) {
+      std::vector<int> U = Q;
+      for (int b : hit) U.insert(U.end(), open[b].q.begin(), open[b].q.end());
+      std::sort(U.begin(), U.end());
+      U.erase(std::unique(U.begin(), U.end()), U.end());
+      if (int(U.size()) <= limit) {
+        Group g;
+        g.q = U;
+        for (int b : hit) g.ops.insert(g.ops.end(), open[b].ops.begin(), open[b].ops.end());
+        g.ops.push_back(i);
+        std::sort(g.ops.begin(), g.ops.end());
+        std::vector<int> rm = hit;
+        std::sort(rm.rbegin(), rm.rend());
+        for (int b : rm) open.erase(open.begin() + b);
+        open.push_back(std::move(g));
+        break;
+      }
+      // close the largest intersecting block and retry
+      auto it = std::max_element(hit.begin(), hit.end(),
+                                 [&](int a, int b) { return open[a].q.size() < open[b].q.size(); });
+      const int b = *it;
+      emitted.push_back(open[b]);
+      open.erase(open.begin() + b);
+      hit.erase(it);
+      for (int& h : hit)
+        if (h > b) --h;
+    }
+  }
+  for (const Group& g : open) emitted.push_back(g);
+  // pack consecutive blocks while their union fits
+  std::vector<Group> phases;
+  for (const Group& g : emitted) {
+    if (!phases.empty()) {
+      std::vector<int> U = phases.back().q;
+      U.insert(U.end(), g.q.begin(), g.q.end());
+      std::sort(U.begin(), U.end());
+      U.erase(std::unique(U.begin(), U.end()), U.end());
+      if (int(U.size()) <= limit) {
+        phases.back().q = U;
+        phases.back().ops.insert(phases.back().ops.end(), g.ops.begin(), g.ops.end());
+        continue;
+      }
+    }
+    phases.push_back(g);
+  }
+  return phases;
+}
+
+void set_c(QOp& o, int i, cd z) {
+  o.m[2 * i] = z.real();
+  o.m[2 * i + 1] = z.imag();
+}
+
+bool is_diag_kind(int k) {
+  return k == K_DIAG1 || k == K_PHASE1 || k == K_CPHASE || k == K_DIAG2;
+}
+
+}  // namespace
+
+int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out) {
+  constexpr int T = kQTileBits, RB = kQRegBits, NR = 1 << kQRegBits;
+  if (n < T) return fail(QSV_EUNSUP, "tiled execution needs at least " + std::to_string(T) + " qubits");
+  for (const KOp& op : in)
+    if (op.kind == K_DENSEK || op.kind == K_DIAGK || op.nq > 2)
+      return fail(QSV_EINVAL, "tiled execution takes unfused gates");
+  out = QProgram();
+  out.n = n;
+  std::vector<KOp> rem = merge_one_qubit_runs(in, n);
+  out.gates_absorbed = in.size();
+  const size_t window = 512;  // ops considered when choosing a tile
+  std::vector<char> inS(n), blocked(n);
+  while (!rem.empty()) {
+    // ---- tile qubits: the line bits, the first pending op, then greedily the
+    // qubit that lets the tile absorb the most pending ops
+    std::fill(inS.begin(), inS.end(), 0);
+    int size = 0;
+    auto add = [&](int q) {
+      if (!inS[q]) {
+        inS[q] = 1;
+        ++size;
+      }
+    };
+    for (int j = 0; j < line_bits && j < n; ++j) add(n - 1 - j);
+    for (int j = 0; j < rem[0].nq; ++j) add(rem[0].q[j]);
+    while (size < T) {
+      std::set<int> cand;
+      for (size_t i = 0; i < std::min(window, rem.size()); ++i)
+        for (int j = 0; j < rem[i].nq; ++j)
+          if (!inS[rem[i].q[j]]) cand.insert(rem[i].q[j]);
+      if (cand.empty()) {
+        for (int q = 0; q < n && size < T; ++q) add(q);
+        break;
+      }
+      int best = -1, bq = -1;
+      for (int q : cand) {
+        inS[q] = 1;
+        const int a = count_absorbed(rem, window, inS, blocked);
+        inS[q] = 0;
+        if (a > best) {
+          best = a;
+          bq = q;
+        }
+      }
+      add(bq);
+    }
+    // ---- absorb
+    std::vector<KOp> mine, deferred;
+    std::fill(blocked.begin(), blocked.end(), 0);
+    for (const KOp& op : rem) {
+      bool ok = true;
+      for (int j = 0; j < op.nq; ++j) ok = ok && inS[op.q[j]] && !blocked[op.q[j]];
+      if (ok) {
+        mine.push_back(op);
+      } else {
+        for (int j = 0; j < op.nq; ++j) blocked[op.q[j]] = 1;
+        deferred.push_back(op);
+      }
+    }
+    if (mine.empty()) return fail(QSV_EINVAL, "tiled compile made no progress");
+    // ---- pass descriptor
+    QPass P{};
+    std::vector<int> tb;  // tile address bits ascending
+    for (int q = 0; q < n; ++q)
+      if (inS[q]) tb.push_back(n - 1 - q);
+    std::sort(tb.begin(), tb.end());
+    for (int i = 0; i < T; ++i) P.tbits[i] = uint8_t(tb[i]);
+    uint64_t tmask = 0;
+    for (int b : tb) tmask |= uint64_t(1) << b;
+    P.nruns = 0;
+    for (int b = 0; b < n;) {
+      if (tmask >> b & 1) {
+        ++b;
+        continue;
+      }
+      int e = b;
+      while (e < n && !(tmask >> e & 1)) ++e;
+      P.rdst[P.nruns] = uint8_t(b);
+      P.rlen[P.nruns] = uint8_t(e - b);
+      ++P.nruns;
+      b = e;
+    }
+    P.ntiles = uint64_t(1) << (n - T);
+    auto local = [&](int q) {  // tile-local bit of qubit q
+      return int(std::lower_bound(tb.begin(), tb.end(), n - 1 - q) - tb.begin());
+    };
+    // ---- phases
+    P.phase_begin = int(out.phases.size());
+    for (const Group& g : make_phases(mine, RB)) {
+      QPhase ph{};
+      uint32_t regmask = 0;
+      for (int q : g.q) regmask |= 1u << local(q);
+      // pad with the highest free tile bits: the low tile bits then index
+      // the lanes, so a wave's LDS accesses stay spread over the banks
+      for (int b = T - 1; b >= 0 && __builtin_popcount(regmask) < RB; --b) regmask |= 1u << b;
+      ph.regmask = regmask;
+      std::vector<int> rbit;  // register index bit r <-> tile-local bit rbit[r]
+      for (int b = 0; b < T; ++b)
+        if (regmask >> b & 1) rbit.push_back(b);
+      for (int i = 0; i < NR; ++i) {
+        uint32_t pos = 0;
+        for (int r = 0; r < RB; ++r)
+          if (i >> r & 1) pos |= 1u << rbit[r];
+        ph.regpos[i] = uint16_t(qtile_swizzle(pos));
+      }
+      auto rb = [&](int q) {
+        const int L = local(q);
+        return int(std::find(rbit.begin(), rbit.end(), L) - rbit.begin());
+      };
+      ph.op_begin = int(out.ops.size());
+      bool open_diag = false;  // the last emitted op is a diagonal table still open for merging
+      for (int oi : g.ops) {
+        const KOp& op = mine[oi];
+        if (is_diag_kind(op.kind)) {
+          if (!open_diag) {
+            QOp d{};
+            d.kind = Q_DIAGR;
+            for (int i = 0; i < NR; ++i) set_c(d, i, 1.0);
+            out.ops.push_back(d);
+            open_diag = true;
+          }
+          QOp& d = out.ops.back();
+          for (int i = 0; i < NR; ++i) {
+            auto bit = [&](int q) { return (i >> rb(q)) & 1; };
+            cd f = 1.0;
+            switch (op.kind) {
+              case K_DIAG1: f = bit(op.q[0]) ? op.m[1] : op.m[0]; break;
+              case K_PHASE1: f = bit(op.q[0]) ? op.m[0] : cd(1.0); break;
+              case K_CPHASE: f = (bit(op.q[0]) == op.cs && bit(op.q[1])) ? op.m[0] : cd(1.0); break;
+              default: f = op.m[2 * bit(op.q[0]) + bit(op.q[1])]; break;  // K_DIAG2
+            }
+            const cd cur(d.m[2 * i], d.m[2 * i + 1]);
+            set_c(d, i, cur * f);
+          }
+          continue;
+        }
+        open_diag = false;
+        QOp o{};
+        switch (op.kind) {
+          case K_DENSE1: {
+            const cd* m = op.m;
+            const bool real = m[0].imag() == 0 && m[1].imag() == 0 && m[2].imag() == 0 &&
+                              m[3].imag() == 0;
+            const bool xlike = m[0].imag() == 0 && m[3].imag() == 0 && m[1].real() == 0 &&
+                               m[2].real() == 0;
+            o.kind = real ? Q_REAL1 : (xlike ? Q_XLIKE1 : Q_DENSE1);
+            o.a = rb(op.q[0]);
+            for (int i = 0; i < 4; ++i) set_c(o, i, op.m[i]);
+            break;
+          }
+          case K_CDENSE1: {
+            const bool x = op.m[0] == cd(0.0) && op.m[1] == cd(1.0) && op.m[2] == cd(1.0) &&
+                           op.m[3] == cd(0.0);
+            o.kind = x ? Q_CX : Q_CDENSE1;
+            o.a = rb(op.q[1]);
+            o.b = rb(op.q[0]);
+            o.cs = op.cs;
+            for (int i = 0; i < 4; ++i) set_c(o, i, op.m[i]);
+            break;
+          }
+          case K_SWAP:
+            o.kind = Q_SWAP;
+            o.a = rb(op.q[0]);
+            o.b = rb(op.q[1]);
+            set_c(o, 0, op.m[0]);
+            break;
+          case K_DENSE2:
+            o.kind = Q_DENSE2;
+            o.a = rb(op.q[0]);
+            o.b = rb(op.q[1]);
+            for (int i = 0; i < 16; ++i) set_c(o, i, op.m[i]);
+            break;
+          default:
+            return fail(QSV_EUNSUP, "tiled execution: kernel kind " + std::to_string(op.kind));
+        }
+        out.ops.push_back(o);
+      }
+      ph.op_end = int(out.ops.size());
+      for (int o = ph.op_begin; o < ph.op_end; ++o) {
+        QOp& d = out.ops[o];
+        if (d.kind != Q_DIAGR) continue;
+        d.b = 0;
+        for (int i = 0; i < NR; ++i)
+          if (!(d.m[2 * i] == 1.0 && d.m[2 * i + 1] == 0.0)) d.b |= 1 << i;
+      }
+      out.phases.push_back(ph);
+    }
+    P.phase_end = int(out.phases.size());
+    out.passes.push_back(P);
+    rem.swap(deferred);
+  }
+  return QSV_OK;
+}
+
+}  // namespace qsv

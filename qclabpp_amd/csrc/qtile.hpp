@@ -1,0 +1,89 @@
+// Tiled execution of one large circuit: each launch ("pass") streams the
+// state through LDS once, in tiles of 2^T amplitudes (T address bits, the
+// lowest line bits always among them), and applies every gate of the pass
+// to the tile.  Inside a pass the gates are grouped into phases: a phase
+// moves each thread's 2^RB amplitudes (RB tile bits, the phase's qubits)
+// from LDS into registers, applies its gates there one by one on the VALU,
+// and writes them back.  A phase costs one LDS round trip; a pass costs one
+// HBM round trip.
+//
+// No counterpart in the reference, which sweeps the state once per gate
+// (include/qclab/QCircuit.hpp:130-136, src/qgates/apply.hpp:258-371); the
+// result is the same product of the same lowered gates.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace qsv {
+
+constexpr int kQTileBits = 12;     // tile: 2^12 amplitudes (64 KiB of complex<double>)
+constexpr int kQRegBits = 4;       // registers: 16 amplitudes per thread
+constexpr int kQThreads = 256;     // (1 << kQTileBits) / (1 << kQRegBits)
+
+enum QKind : int32_t {
+  Q_DENSE1 = 1,   // 2x2 on register bit a                        m[0..3] row-major
+  Q_DIAGR = 2,    // v[i] *= m[i] for the 16 register indices     m[0..15]
+  Q_CDENSE1 = 3,  // 2x2 on register bit a where bit b == cs      m[0..3]
+  Q_CX = 4,       // exchange over bit a where bit b == cs
+  Q_SWAP = 5,     // exchange |a=1,b=0> and |a=0,b=1>, times m[0]
+  Q_DENSE2 = 6,   // 4x4 on register bits (a, b), row = 2*bit(a) + bit(b)   m[0..15]
+  Q_REAL1 = 7,    // 2x2 with real entries on bit a (RotationY)   re(m[0..3])
+  Q_XLIKE1 = 8,   // [[p, iq], [ir, s]], p q r s real (RotationX) re(m0) im(m1) im(m2) re(m3)
+};
+// Q_DIAGR: field b holds the mask of register indices whose factor is not 1.
+
+struct QOp {
+  int32_t kind;
+  int32_t a, b, cs;
+  double m[32];   // complex coefficients, (re, im) interleaved
+};
+
+struct QPhase {
+  uint32_t regmask;                 // tile-local bits held in registers
+  uint16_t regpos[1 << kQRegBits];  // LDS slot part of register index i (swizzled)
+  int32_t op_begin, op_end;
+};
+
+// LDS slot of tile position p: the low 4 bits are XOR-ed with the two
+// nibbles above them, so 16 lanes that differ in any of the three nibbles'
+// bits mostly land in different bank groups.  XOR-linear: the slot of
+// (a | b), a and b without common bits, is qtile_swizzle(a) ^ qtile_swizzle(b).
+inline __host__ __device__ uint32_t qtile_swizzle(uint32_t p) {
+  return p ^ (((p >> 4) ^ (p >> 8)) & 15u);
+}
+
+struct QPass {
+  int32_t phase_begin, phase_end;
+  int32_t nruns;
+  uint8_t tbits[16];    // global address bit of tile-local bit i, ascending
+  uint8_t rdst[24];     // runs of address bits outside the tile
+  uint8_t rlen[24];
+  uint64_t ntiles;
+};
+
+struct QProgram {
+  int n = 0;
+  std::vector<QPass> passes;
+  std::vector<QPhase> phases;
+  std::vector<QOp> ops;
+  size_t gates_absorbed = 0;
+};
+
+// Compile lowered, unfused gates (n > kQTileBits) into passes and phases.
+// line_bits: address bits inside one 128-byte line, always kept in the tile.
+int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out);
+
+struct QTileLaunch {
+  void* psi;
+  const QPass* pass;      // device
+  const QPhase* phases;   // device
+  const QOp* ops;         // device
+  uint64_t ntiles;
+};
+
+template <typename R>
+int launch_qtile(const QTileLaunch& a, hipStream_t st);
+
+}  // namespace qsv

@@ -97,8 +97,12 @@ def simulate_host(vec: np.ndarray, gates) -> np.ndarray:
     return out
 
 
+TILED = _lib.QSV_TILED
+
+
 class Program:
-    """A compiled circuit (qsv_program): plain (fuse_k=0) or fused (1..5)."""
+    """A compiled circuit (qsv_program): plain (fuse_k=0), fused (1..5) or
+    tiled passes (fuse_k=TILED)."""
 
     def __init__(self, n: int, gates, fuse_k: int = 5, dtype: str = "c128", diag_k: int = 0):
         self.n = n
@@ -126,7 +130,7 @@ class Program:
         check(lib().qsv_program_info_get(self._h, C.byref(inf)))
         return {"ngates": inf.ngates, "nlaunch": inf.nlaunch, "nplain": inf.nplain,
                 "ndiag": inf.ndiag, "ndense": list(inf.ndense), "alg_bytes": inf.alg_bytes,
-                "flops": inf.flops}
+                "flops": inf.flops, "tiled_phases": inf.ntiled_phases}
 
     def launches(self):
         """Every launch of one run, in order: (qubits, diag, coefficients) with
