@@ -36,11 +36,14 @@ inline void check(const int status) {
   if (status != QSV_OK) throw error(status, std::string("qclab MI355X engine: ") + qsv_last_error());
 }
 
-/// Fusion width used by QCircuit::simulate/apply (environment QCLAB_FUSE_K,
-/// default 5; 0 applies one plain sweep per gate).
+/// Compiled form used by QCircuit::simulate/apply: environment QCLAB_FUSE_K =
+/// 0 (one plain sweep per gate), 1..5 (fused blocks of up to k qubits),
+/// "tiled" (tiled passes) or "auto" (the default: tiled from 22 qubits up,
+/// fused k = 5 below).
 inline int fuse_k() {
   const char* e = std::getenv("QCLAB_FUSE_K");
-  if (!e) return 5;
+  if (!e || std::string(e) == "auto") return QSV_AUTO;
+  if (std::string(e) == "tiled") return QSV_TILED;
   const int k = std::atoi(e);
   return k < 0 ? 0 : (k > 5 ? 5 : k);
 }

@@ -177,6 +177,9 @@ int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates,
  * qubits use fuse_k = 5 instead).  Coefficients live on the device for the
  * program's lifetime; running it allocates nothing.                       */
 #define QSV_TILED (-1)
+/* fuse_k == QSV_AUTO picks QSV_TILED from 22 qubits up (at least 1024 tiles
+ * to spread over the GPU) and fuse_k = 5 below.                          */
+#define QSV_AUTO  (-2)
 typedef struct qsv_program qsv_program;
 
 typedef struct qsv_program_info {
@@ -295,7 +298,8 @@ int qsv_dev_max_abs_diff(const void* a, const void* b, uint64_t count, int dtype
 int qsv_host_apply_gates(void* host, int n, int dtype, const qsv_gate* gates,
                          size_t ngates);
 /* Same, running the gate list as one compiled program with fusion width
- * fuse_k (0 = plain launches, 1..5 = fused, QSV_TILED = tiled passes).   */
+ * fuse_k (0 = plain launches, 1..5 = fused, QSV_TILED = tiled passes,
+ * QSV_AUTO = the engine's choice).                                        */
 int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_t ngates,
                       int fuse_k);
 /* Device-resident variant: compile, run on `psi`, synchronise `stream`.   */

@@ -71,6 +71,7 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
   ngates_in = ngates;
   std::vector<KOp> lowered(ngates);
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, lowered[i]));
+  if (fuse_k == QSV_AUTO) fuse_k = n >= kQTileBits + 10 ? QSV_TILED : kMaxFuse;
   if (fuse_k == QSV_TILED && n < kQTileBits) fuse_k = kMaxFuse;  // one tile would be the whole state
   if (fuse_k == QSV_TILED) {
     tiled = true;

