@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--adjoint-n", type=int, default=24,
                     help="qubits of the adjoint-vs-finite-difference measurement (0 skips it)")
     ap.add_argument("--adjoint-layers", type=int, default=10)
+    ap.add_argument("--adjoint-big-n", type=int, default=30,
+                    help="qubits of the memory-scale adjoint run (three 2^n states; 0 skips it)")
     ap.add_argument("--ps-params", type=int, default=1,
                     help="parameters checked against the CPU parameter-shift gradient")
     ap.add_argument("--batch", type=int, default=512,
@@ -138,6 +140,43 @@ def adjoint_leg(args):
         out["param_shift_cpu"] = {"params": which, "s": time.perf_counter() - t0,
                                   "max_abs_adjoint_minus_ps": float(np.abs(g_adj[which] - ps).max())}
     return out
+
+
+def adjoint_big_leg(args):
+    """The same ansatz and Hamiltonian at n=30 (two complex128 states of
+    16 GiB: psi and lambda = H psi; the reverse sweep recomputes states by
+    inversion instead of storing them): adjoint time and energy, with two
+    parameters checked against central differences."""
+    import numpy as np
+    import torch
+
+    from qclabpp_amd import engine as E
+    from qclabpp_amd import observables as O
+    from qclabpp_amd import workloads as W
+    from qclabpp_amd.autograd import finite_difference
+
+    n, L = args.adjoint_big_n, args.adjoint_layers
+    circ = W.tfim_ansatz(n, L)
+    theta = np.array(W.params_of(circ))
+    H = O.tfim(n)
+    psi = E.new_state(n)
+    lam = torch.empty_like(psi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e_adj, g_adj = O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    del psi, lam
+    torch.cuda.empty_cache()
+    which = [0, theta.size - 1]
+    t0 = time.perf_counter()
+    g_fd = finite_difference(circ, H, theta, n, h=1e-6, fuse_k=E.AUTO, which=which)
+    t_fd = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    return {"workload": f"tfim ansatz n={n} layers={L} params={theta.size} complex128",
+            "state_bytes": (1 << n) * 16, "energy": e_adj, "adjoint_s": t_adj,
+            "fd_params": which, "fd_s_for_those": t_fd,
+            "max_abs_adjoint_minus_fd": float(np.abs(g_adj[which] - g_fd).max())}
 
 
 def batch_leg(args):
@@ -418,6 +457,9 @@ def main():
     torch.cuda.empty_cache()
     if args.adjoint_n > 0:
         out["adjoint"] = adjoint_leg(args)
+        torch.cuda.empty_cache()
+    if args.adjoint_big_n > 0:
+        out["adjoint_big"] = adjoint_big_leg(args)
         torch.cuda.empty_cache()
     if args.batch > 0:
         out["batch"] = batch_leg(args)
