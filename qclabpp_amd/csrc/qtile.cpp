@@ -146,11 +146,30 @@ std::vector<Group> make_phases(const std::vector<KOp>& ops, int limit) {
 void set_c(QOp& o, int i, cd z) {
   o.m[2 * i] = z.real();
   o.m[2 * i + 1] = z.imag();
+  o.mf[2 * i] = float(z.real());
+  o.mf[2 * i + 1] = float(z.imag());
 }
 
-bool is_diag_kind(int k) {
-  return k == K_DIAG1 || k == K_PHASE1 || k == K_CPHASE || k == K_DIAG2;
-}
+// Which register holds which logical amplitude inside a phase.  Only
+// permutations a'[j] = a[pi(j)] with pi affine over GF(2) are applied (X,
+// CX, SWAP and their controlled-state variants), so phys stays affine:
+// phys[j] = L*j ^ phys[0].
+struct Frame {
+  uint8_t phys[1 << kQRegBits];  // register of logical index j
+  uint8_t lg[1 << kQRegBits];    // logical index held by register p
+  void reset() {
+    for (int j = 0; j < (1 << kQRegBits); ++j) phys[j] = lg[j] = uint8_t(j);
+  }
+  template <typename F>
+  void permute(F pi) {
+    uint8_t np[1 << kQRegBits];
+    for (int j = 0; j < (1 << kQRegBits); ++j) np[j] = phys[pi(j)];
+    for (int j = 0; j < (1 << kQRegBits); ++j) {
+      phys[j] = np[j];
+      lg[np[j]] = uint8_t(j);
+    }
+  }
+};
 
 }  // namespace
 
@@ -243,103 +262,173 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
     // ---- phases
     P.phase_begin = int(out.phases.size());
     for (const Group& g : make_phases(mine, RB)) {
-      QPhase ph{};
       uint32_t regmask = 0;
       for (int q : g.q) regmask |= 1u << local(q);
       // pad with the highest free tile bits: the low tile bits then index
       // the lanes, so a wave's LDS accesses stay spread over the banks
       for (int b = T - 1; b >= 0 && __builtin_popcount(regmask) < RB; --b) regmask |= 1u << b;
-      ph.regmask = regmask;
-      std::vector<int> rbit;  // register index bit r <-> tile-local bit rbit[r]
+      std::vector<int> rbit;  // logical register bit r <-> tile-local bit rbit[r]
       for (int b = 0; b < T; ++b)
         if (regmask >> b & 1) rbit.push_back(b);
-      for (int i = 0; i < NR; ++i) {
+      auto slot = [&](int i) {  // LDS slot part of logical register index i
         uint32_t pos = 0;
         for (int r = 0; r < RB; ++r)
           if (i >> r & 1) pos |= 1u << rbit[r];
-        ph.regpos[i] = uint16_t(qtile_swizzle(pos));
-      }
+        return uint16_t(qtile_swizzle(pos));
+      };
       auto rb = [&](int q) {
         const int L = local(q);
         return int(std::find(rbit.begin(), rbit.end(), L) - rbit.begin());
       };
-      ph.op_begin = int(out.ops.size());
-      bool open_diag = false;  // the last emitted op is a diagonal table still open for merging
+      Frame fr;
+      cd diag[NR];
+      bool pending = false;  // diag holds factors not yet emitted
+      QPhase ph{};
+      auto open_phase = [&]() {
+        ph = QPhase{};
+        ph.regmask = regmask;
+        for (int i = 0; i < NR; ++i) ph.regpos[i] = slot(i);
+        ph.op_begin = int(out.ops.size());
+        fr.reset();
+        for (int i = 0; i < NR; ++i) diag[i] = 1.0;
+        pending = false;
+      };
+      auto flush_diag = [&]() {
+        if (!pending) return;
+        QOp d{};
+        d.kind = Q_DIAGR;
+        for (int p = 0; p < NR; ++p) {
+          set_c(d, p, diag[p]);
+          if (diag[p] != cd(1.0)) d.b |= 1 << p;
+          diag[p] = 1.0;
+        }
+        if (d.b) out.ops.push_back(d);
+        pending = false;
+      };
+      auto close_phase = [&]() {
+        flush_diag();
+        for (int p = 0; p < NR; ++p) ph.regpos_out[p] = slot(fr.lg[p]);
+        ph.op_end = int(out.ops.size());
+        out.phases.push_back(ph);
+      };
+      // multiply register p by f(logical index of p)
+      auto scale = [&](auto f) {
+        for (int p = 0; p < NR; ++p) {
+          const cd z = f(int(fr.lg[p]));
+          if (z != cd(1.0)) {
+            diag[p] *= z;
+            pending = true;
+          }
+        }
+      };
+      open_phase();
       for (int oi : g.ops) {
         const KOp& op = mine[oi];
-        if (is_diag_kind(op.kind)) {
-          if (!open_diag) {
-            QOp d{};
-            d.kind = Q_DIAGR;
-            for (int i = 0; i < NR; ++i) set_c(d, i, 1.0);
-            out.ops.push_back(d);
-            open_diag = true;
-          }
-          QOp& d = out.ops.back();
-          for (int i = 0; i < NR; ++i) {
-            auto bit = [&](int q) { return (i >> rb(q)) & 1; };
-            cd f = 1.0;
-            switch (op.kind) {
-              case K_DIAG1: f = bit(op.q[0]) ? op.m[1] : op.m[0]; break;
-              case K_PHASE1: f = bit(op.q[0]) ? op.m[0] : cd(1.0); break;
-              case K_CPHASE: f = (bit(op.q[0]) == op.cs && bit(op.q[1])) ? op.m[0] : cd(1.0); break;
-              default: f = op.m[2 * bit(op.q[0]) + bit(op.q[1])]; break;  // K_DIAG2
-            }
-            const cd cur(d.m[2 * i], d.m[2 * i + 1]);
-            set_c(d, i, cur * f);
-          }
-          continue;
-        }
-        open_diag = false;
-        QOp o{};
+        const int t0 = rb(op.q[0]);
+        const int t1 = op.nq > 1 ? rb(op.q[1]) : -1;
+        auto bit = [](int i, int r) { return (i >> r) & 1; };
         switch (op.kind) {
-          case K_DENSE1: {
-            const cd* m = op.m;
-            const bool real = m[0].imag() == 0 && m[1].imag() == 0 && m[2].imag() == 0 &&
-                              m[3].imag() == 0;
-            const bool xlike = m[0].imag() == 0 && m[3].imag() == 0 && m[1].real() == 0 &&
-                               m[2].real() == 0;
-            o.kind = real ? Q_REAL1 : (xlike ? Q_XLIKE1 : Q_DENSE1);
-            o.a = rb(op.q[0]);
-            for (int i = 0; i < 4; ++i) set_c(o, i, op.m[i]);
-            break;
-          }
-          case K_CDENSE1: {
-            const bool x = op.m[0] == cd(0.0) && op.m[1] == cd(1.0) && op.m[2] == cd(1.0) &&
-                           op.m[3] == cd(0.0);
-            o.kind = x ? Q_CX : Q_CDENSE1;
-            o.a = rb(op.q[1]);
-            o.b = rb(op.q[0]);
-            o.cs = op.cs;
-            for (int i = 0; i < 4; ++i) set_c(o, i, op.m[i]);
-            break;
-          }
+          case K_DIAG1:
+            scale([&](int i) { return bit(i, t0) ? op.m[1] : op.m[0]; });
+            continue;
+          case K_PHASE1:
+            scale([&](int i) { return bit(i, t0) ? op.m[0] : cd(1.0); });
+            continue;
+          case K_CPHASE:
+            scale([&](int i) { return bit(i, t0) == op.cs && bit(i, t1) ? op.m[0] : cd(1.0); });
+            continue;
+          case K_DIAG2:
+            scale([&](int i) { return op.m[2 * bit(i, t0) + bit(i, t1)]; });
+            continue;
           case K_SWAP:
-            o.kind = Q_SWAP;
-            o.a = rb(op.q[0]);
-            o.b = rb(op.q[1]);
-            set_c(o, 0, op.m[0]);
-            break;
-          case K_DENSE2:
+            fr.permute([&](int j) {
+              return bit(j, t0) == bit(j, t1) ? j : j ^ (1 << t0) ^ (1 << t1);
+            });
+            scale([&](int i) { return bit(i, t0) != bit(i, t1) ? op.m[0] : cd(1.0); });
+            continue;
+          case K_DENSE1:
+          case K_CDENSE1: {
+            const bool ctl = op.kind == K_CDENSE1;
+            const int t = ctl ? t1 : t0, c = ctl ? t0 : -1;
+            auto on = [&](int i) { return !ctl || bit(i, c) == op.cs; };
+            const cd* m = op.m;
+            if (m[1] == cd(0.0) && m[2] == cd(0.0)) {  // diagonal
+              scale([&](int i) { return on(i) ? (bit(i, t) ? m[3] : m[0]) : cd(1.0); });
+              continue;
+            }
+            if (m[0] == cd(0.0) && m[3] == cd(0.0)) {  // antidiagonal: relabel, then scale
+              fr.permute([&](int j) { return on(j) ? j ^ (1 << t) : j; });
+              scale([&](int i) { return on(i) ? (bit(i, t) ? m[2] : m[1]) : cd(1.0); });
+              continue;
+            }
+            flush_diag();
+            QOp o{};
+            const int d = fr.phys[1 << t] ^ fr.phys[0];
+            const int K = d & -d;
+            o.d = d;
+            int nbase = 0;
+            for (int p = 0; p < NR; ++p) {
+              if (p & K) continue;
+              if (int(fr.lg[p ^ d]) != (fr.lg[p] ^ (1 << t)))
+                return fail(QSV_EINVAL, "tiled compile: inconsistent register relabeling");
+              ++nbase;
+              if (bit(fr.lg[p], t)) o.roles |= 1u << p;
+              if (on(fr.lg[p])) o.act |= 1u << p;
+            }
+            cd c4[4] = {m[0], m[1], m[2], m[3]};
+            if (__builtin_popcount(o.roles) == nbase) {  // every base holds the |1> side
+              c4[0] = m[3]; c4[1] = m[2]; c4[2] = m[1]; c4[3] = m[0];
+              o.roles = 0;
+            }
+            if (c4[0] == c4[3] && c4[1] == c4[2]) o.roles = 0;  // symmetric under the swap
+            for (int i = 0; i < 4; ++i) set_c(o, i, c4[i]);
+            const bool real = c4[0].imag() == 0 && c4[1].imag() == 0 && c4[2].imag() == 0 &&
+                              c4[3].imag() == 0;
+            const bool xlike = c4[0].imag() == 0 && c4[3].imag() == 0 && c4[1].real() == 0 &&
+                               c4[2].real() == 0;
+            if (ctl) o.kind = Q_CPAIR;
+            else if (real && c4[0] == c4[3] && c4[2] == -c4[1]) o.kind = Q_ROTP;
+            else if (real) o.kind = Q_REALP;
+            else if (xlike) o.kind = Q_XLIKEP;
+            else o.kind = Q_PAIR;
+            out.ops.push_back(o);
+            continue;
+          }
+          case K_DENSE2: {
+            // needs each of the two logical bits on one register bit, up to
+            // a constant flip; otherwise start a new phase (identity relabeling)
+            auto unit_bit = [&](int t, int& x, int& flip) {
+              const int d = fr.phys[1 << t] ^ fr.phys[0];
+              if (__builtin_popcount(d) != 1) return false;
+              x = __builtin_ctz(d);
+              flip = bit(fr.lg[0], t);
+              for (int p = 0; p < NR; ++p)
+                if (bit(fr.lg[p], t) != (bit(p, x) ^ flip)) return false;
+              return true;
+            };
+            int xa, fa, xb, fb;
+            if (!unit_bit(t0, xa, fa) || !unit_bit(t1, xb, fb)) {
+              close_phase();
+              open_phase();
+              xa = t0; fa = 0;
+              xb = t1; fb = 0;
+            }
+            flush_diag();
+            QOp o{};
             o.kind = Q_DENSE2;
-            o.a = rb(op.q[0]);
-            o.b = rb(op.q[1]);
-            for (int i = 0; i < 16; ++i) set_c(o, i, op.m[i]);
-            break;
+            o.a = xa;
+            o.b = xb;
+            const int s = 2 * fa + fb;
+            for (int r = 0; r < 4; ++r)
+              for (int c = 0; c < 4; ++c) set_c(o, 4 * r + c, op.m[4 * (r ^ s) + (c ^ s)]);
+            out.ops.push_back(o);
+            continue;
+          }
           default:
             return fail(QSV_EUNSUP, "tiled execution: kernel kind " + std::to_string(op.kind));
         }
-        out.ops.push_back(o);
       }
-      ph.op_end = int(out.ops.size());
-      for (int o = ph.op_begin; o < ph.op_end; ++o) {
-        QOp& d = out.ops[o];
-        if (d.kind != Q_DIAGR) continue;
-        d.b = 0;
-        for (int i = 0; i < NR; ++i)
-          if (!(d.m[2 * i] == 1.0 && d.m[2 * i + 1] == 0.0)) d.b |= 1 << i;
-      }
-      out.phases.push_back(ph);
+      close_phase();
     }
     P.phase_end = int(out.phases.size());
     out.passes.push_back(P);

@@ -31,25 +31,22 @@ __device__ __forceinline__ uint32_t pdep12(uint32_t x, uint32_t mask) {
   return r;
 }
 
+// Coefficient i of an op, from the table of the state's precision.
 template <typename C>
 __device__ __forceinline__ C qc(const QCONST QOp& o, int i) {
-  using R = decltype(C::x);
-  return mk<C>(R(o.m[2 * i]), R(o.m[2 * i + 1]));
+  if constexpr (sizeof(C) == sizeof(double2))
+    return mk<C>(o.m[2 * i], o.m[2 * i + 1]);
+  else
+    return mk<C>(o.mf[2 * i], o.mf[2 * i + 1]);
+}
+template <typename C>
+__device__ __forceinline__ decltype(C::x) qre(const QCONST QOp& o, int i) {
+  if constexpr (sizeof(C) == sizeof(double2)) return o.m[i];
+  else return o.mf[i];
 }
 
-template <int J, typename C>
-__device__ __forceinline__ void q_dense1(C (&v)[NR], C m0, C m1, C m2, C m3) {
-#pragma unroll
-  for (int i = 0; i < NR; ++i)
-    if (!(i & (1 << J))) {
-      const C x = v[i], y = v[i | (1 << J)];
-      v[i] = cmad(m1, y, cmul(m0, x));
-      v[i | (1 << J)] = cmad(m3, y, cmul(m2, x));
-    }
-}
-
-// The control state is a run-time value: results are chosen with value
-// selects, never with conditional stores, so v stays in registers.
+// Results are chosen with value selects, never with conditional stores, so
+// v stays in registers.  Selects on wave-uniform values stay scalar.
 template <typename C>
 __device__ __forceinline__ C qsel(bool c, C a, C b) {
   C r;
@@ -58,69 +55,98 @@ __device__ __forceinline__ C qsel(bool c, C a, C b) {
   return r;
 }
 
-template <int TB, int CB, typename C>
-__device__ __forceinline__ void q_cdense1(C (&v)[NR], C m0, C m1, C m2, C m3, int cs) {
+// Pair ops over the register pairs (p, p ^ D), p the base (bit lowbit(D)
+// clear).  `roles` bit p: base p holds the |1> side, so the 2x2 runs with
+// rows and columns swapped; the choice is a select on uniform coefficients
+// (scalar instructions), compiled out (SEL false) for ops with no roles.
+template <int D, bool SEL, typename C>
+__device__ __forceinline__ void q_pair(C (&v)[NR], const QCONST QOp& o) {
+  constexpr int K = D & -D;
+  const C m0 = qc<C>(o, 0), m1 = qc<C>(o, 1), m2 = qc<C>(o, 2), m3 = qc<C>(o, 3);
+  const uint32_t roles = o.roles;
 #pragma unroll
-  for (int i = 0; i < NR; ++i)
-    if (!(i & (1 << TB))) {
-      const bool on = ((i >> CB) & 1) == cs;
-      const C x = v[i], y = v[i | (1 << TB)];
-      v[i] = qsel(on, cmad(m1, y, cmul(m0, x)), x);
-      v[i | (1 << TB)] = qsel(on, cmad(m3, y, cmul(m2, x)), y);
+  for (int p = 0; p < NR; ++p)
+    if (!(p & K)) {
+      const bool sw = SEL && (roles >> p & 1);
+      const C a = qsel(sw, m3, m0), b = qsel(sw, m2, m1), c = qsel(sw, m1, m2), d = qsel(sw, m0, m3);
+      const C x = v[p], y = v[p ^ D];
+      v[p] = cmad(b, y, cmul(a, x));
+      v[p ^ D] = cmad(d, y, cmul(c, x));
     }
 }
 
-template <int TB, int CB, int CS, typename C>
-__device__ __forceinline__ void q_cx(C (&v)[NR]) {
+// Q_PAIR on the pairs whose base has its `act` bit set
+template <int D, typename C>
+__device__ __forceinline__ void q_cpair(C (&v)[NR], const QCONST QOp& o) {
+  constexpr int K = D & -D;
+  const C m0 = qc<C>(o, 0), m1 = qc<C>(o, 1), m2 = qc<C>(o, 2), m3 = qc<C>(o, 3);
+  const uint32_t roles = o.roles, act = o.act;
 #pragma unroll
-  for (int i = 0; i < NR; ++i)
-    if (!(i & (1 << TB)) && ((i >> CB) & 1) == CS) {
-      const C x = v[i];
-      v[i] = v[i | (1 << TB)];
-      v[i | (1 << TB)] = x;
+  for (int p = 0; p < NR; ++p)
+    if (!(p & K)) {
+      const bool sw = roles >> p & 1, on = act >> p & 1;
+      const C a = qsel(sw, m3, m0), b = qsel(sw, m2, m1), c = qsel(sw, m1, m2), d = qsel(sw, m0, m3);
+      const C x = v[p], y = v[p ^ D];
+      v[p] = qsel(on, cmad(b, y, cmul(a, x)), x);
+      v[p ^ D] = qsel(on, cmad(d, y, cmul(c, x)), y);
     }
 }
 
-// real 2x2 (RotationY): 4 FP ops per amplitude
-template <int J, typename C>
-__device__ __forceinline__ void q_real1(C (&v)[NR], decltype(C::x) a, decltype(C::x) b,
-                                        decltype(C::x) c, decltype(C::x) d) {
+// real 2x2 (4 FP ops per amplitude)
+template <int D, bool SEL, typename C>
+__device__ __forceinline__ void q_realp(C (&v)[NR], const QCONST QOp& o) {
+  using R = decltype(C::x);
+  constexpr int K = D & -D;
+  const R m0 = qre<C>(o, 0), m1 = qre<C>(o, 2), m2 = qre<C>(o, 4), m3 = qre<C>(o, 6);
+  const uint32_t roles = o.roles;
 #pragma unroll
-  for (int i = 0; i < NR; ++i)
-    if (!(i & (1 << J))) {
-      const C x = v[i], y = v[i | (1 << J)];
-      v[i] = mk<C>(a * x.x + b * y.x, a * x.y + b * y.y);
-      v[i | (1 << J)] = mk<C>(c * x.x + d * y.x, c * x.y + d * y.y);
+  for (int p = 0; p < NR; ++p)
+    if (!(p & K)) {
+      const bool sw = SEL && (roles >> p & 1);
+      const R a = sw ? m3 : m0, b = sw ? m2 : m1, c = sw ? m1 : m2, d = sw ? m0 : m3;
+      const C x = v[p], y = v[p ^ D];
+      v[p] = mk<C>(a * x.x + b * y.x, a * x.y + b * y.y);
+      v[p ^ D] = mk<C>(c * x.x + d * y.x, c * x.y + d * y.y);
     }
 }
 
-// [[p, iq], [ir, s]] with p, q, r, s real (RotationX): 4 FP ops per amplitude
-template <int J, typename C>
-__device__ __forceinline__ void q_xlike1(C (&v)[NR], decltype(C::x) p, decltype(C::x) q,
-                                         decltype(C::x) r, decltype(C::x) s) {
+// real rotation [[c, -s], [s, c]]: swapping rows and columns negates s
+template <int D, bool SEL, typename C>
+__device__ __forceinline__ void q_rotp(C (&v)[NR], const QCONST QOp& o) {
+  using R = decltype(C::x);
+  constexpr int K = D & -D;
+  const R c = qre<C>(o, 0), s = qre<C>(o, 4);
+  const uint32_t roles = o.roles;
 #pragma unroll
-  for (int i = 0; i < NR; ++i)
-    if (!(i & (1 << J))) {
-      const C x = v[i], y = v[i | (1 << J)];
-      v[i] = mk<C>(p * x.x - q * y.y, p * x.y + q * y.x);
-      v[i | (1 << J)] = mk<C>(s * y.x - r * x.y, s * y.y + r * x.x);
+  for (int p = 0; p < NR; ++p)
+    if (!(p & K)) {
+      const R t = SEL && (roles >> p & 1) ? -s : s;
+      const C x = v[p], y = v[p ^ D];
+      v[p] = mk<C>(c * x.x - t * y.x, c * x.y - t * y.y);
+      v[p ^ D] = mk<C>(t * x.x + c * y.x, t * x.y + c * y.y);
     }
 }
 
-// exchange |A=1,B=0> with |A=0,B=1>, both times lam
-template <int A, int B, typename C>
-__device__ __forceinline__ void q_swap(C (&v)[NR], C lam, bool scale) {
+// [[p, iq], [ir, s]] with p, q, r, s real (4 FP ops per amplitude);
+// swapping rows and columns exchanges p with s and q with r
+template <int D, bool SEL, typename C>
+__device__ __forceinline__ void q_xlikep(C (&v)[NR], const QCONST QOp& o) {
+  using R = decltype(C::x);
+  constexpr int K = D & -D;
+  const R P = qre<C>(o, 0), Q = qre<C>(o, 3), Rr = qre<C>(o, 5), S = qre<C>(o, 6);
+  const uint32_t roles = o.roles;
 #pragma unroll
-  for (int i = 0; i < NR; ++i)
-    if ((i & (1 << A)) && !(i & (1 << B))) {
-      const int j = i ^ (1 << A) ^ (1 << B);
-      const C x = v[i], y = v[j];
-      v[i] = qsel(scale, cmul(lam, y), y);
-      v[j] = qsel(scale, cmul(lam, x), x);
+  for (int p = 0; p < NR; ++p)
+    if (!(p & K)) {
+      const bool sw = SEL && (roles >> p & 1);
+      const R pp = sw ? S : P, qq = sw ? Rr : Q, rr = sw ? Q : Rr, ss = sw ? P : S;
+      const C x = v[p], y = v[p ^ D];
+      v[p] = mk<C>(pp * x.x - qq * y.y, pp * x.y + qq * y.x);
+      v[p ^ D] = mk<C>(ss * y.x - rr * x.y, ss * y.y + rr * x.x);
     }
 }
 
-// 4x4 on (A, B), row index 2*bit(A) + bit(B)
+// 4x4 on register bits (A, B), row index 2*bit(A) + bit(B)
 template <int A, int B, typename C>
 __device__ __forceinline__ void q_dense2(C (&v)[NR], const QCONST QOp& o) {
 #pragma unroll
@@ -142,20 +168,31 @@ __device__ __forceinline__ void q_dense2(C (&v)[NR], const QCONST QOp& o) {
 
 #define QPAIRS(X) \
   X(0, 1) X(0, 2) X(0, 3) X(1, 0) X(1, 2) X(1, 3) X(2, 0) X(2, 1) X(2, 3) X(3, 0) X(3, 1) X(3, 2)
+#define QDISTS(X) \
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
+
+// FN<D, SEL> for the op's distance, with selects only if some base has a role
+#define QBYDIST_SEL(FN, SEL)                                                    \
+  switch (o.d) {                                                                \
+    case 1: FN<1, SEL>(v, o); break;    case 2: FN<2, SEL>(v, o); break;        \
+    case 3: FN<3, SEL>(v, o); break;    case 4: FN<4, SEL>(v, o); break;        \
+    case 5: FN<5, SEL>(v, o); break;    case 6: FN<6, SEL>(v, o); break;        \
+    case 7: FN<7, SEL>(v, o); break;    case 8: FN<8, SEL>(v, o); break;        \
+    case 9: FN<9, SEL>(v, o); break;    case 10: FN<10, SEL>(v, o); break;      \
+    case 11: FN<11, SEL>(v, o); break;  case 12: FN<12, SEL>(v, o); break;      \
+    case 13: FN<13, SEL>(v, o); break;  case 14: FN<14, SEL>(v, o); break;      \
+    case 15: FN<15, SEL>(v, o); break;  default: break;                         \
+  }
+#define QBYDIST(FN)          \
+  if (o.roles) {             \
+    QBYDIST_SEL(FN, true)    \
+  } else {                   \
+    QBYDIST_SEL(FN, false)   \
+  }
 
 template <typename C>
 __device__ __forceinline__ void apply_op(const QCONST QOp& o, C (&v)[NR]) {
   switch (o.kind) {
-    case Q_DENSE1: {
-      const C m0 = qc<C>(o, 0), m1 = qc<C>(o, 1), m2 = qc<C>(o, 2), m3 = qc<C>(o, 3);
-      switch (o.a) {
-        case 0: q_dense1<0>(v, m0, m1, m2, m3); break;
-        case 1: q_dense1<1>(v, m0, m1, m2, m3); break;
-        case 2: q_dense1<2>(v, m0, m1, m2, m3); break;
-        default: q_dense1<3>(v, m0, m1, m2, m3); break;
-      }
-      break;
-    }
     case Q_DIAGR: {
       const int mask = o.b;  // factors other than 1
 #pragma unroll
@@ -163,59 +200,26 @@ __device__ __forceinline__ void apply_op(const QCONST QOp& o, C (&v)[NR]) {
         if (mask >> i & 1) v[i] = cmul(qc<C>(o, i), v[i]);
       break;
     }
-    case Q_REAL1: {
-      using R = decltype(C::x);
-      const R a = R(o.m[0]), b = R(o.m[2]), c = R(o.m[4]), d = R(o.m[6]);
-      switch (o.a) {
-        case 0: q_real1<0>(v, a, b, c, d); break;
-        case 1: q_real1<1>(v, a, b, c, d); break;
-        case 2: q_real1<2>(v, a, b, c, d); break;
-        default: q_real1<3>(v, a, b, c, d); break;
-      }
+    case Q_ROTP:
+      QBYDIST(q_rotp);
       break;
-    }
-    case Q_XLIKE1: {
-      using R = decltype(C::x);
-      const R p = R(o.m[0]), q = R(o.m[3]), r = R(o.m[5]), ss = R(o.m[6]);
-      switch (o.a) {
-        case 0: q_xlike1<0>(v, p, q, r, ss); break;
-        case 1: q_xlike1<1>(v, p, q, r, ss); break;
-        case 2: q_xlike1<2>(v, p, q, r, ss); break;
-        default: q_xlike1<3>(v, p, q, r, ss); break;
-      }
+    case Q_XLIKEP:
+      QBYDIST(q_xlikep);
       break;
-    }
-    case Q_CDENSE1: {
-      const C m0 = qc<C>(o, 0), m1 = qc<C>(o, 1), m2 = qc<C>(o, 2), m3 = qc<C>(o, 3);
-      switch (o.a * 4 + o.b) {
-#define X(A, B) case A * 4 + B: q_cdense1<A, B>(v, m0, m1, m2, m3, o.cs); break;
-        QPAIRS(X)
+    case Q_REALP:
+      QBYDIST(q_realp);
+      break;
+    case Q_PAIR:
+      QBYDIST(q_pair);
+      break;
+    case Q_CPAIR:
+      switch (o.d) {
+#define X(D) case D: q_cpair<D>(v, o); break;
+        QDISTS(X)
 #undef X
         default: break;
       }
       break;
-    }
-    case Q_CX: {
-      switch ((o.a * 4 + o.b) * 2 + o.cs) {
-#define X(A, B) case (A * 4 + B) * 2: q_cx<A, B, 0>(v); break; \
-                case (A * 4 + B) * 2 + 1: q_cx<A, B, 1>(v); break;
-        QPAIRS(X)
-#undef X
-        default: break;
-      }
-      break;
-    }
-    case Q_SWAP: {
-      const C lam = qc<C>(o, 0);
-      const bool scale = !(o.m[0] == 1.0 && o.m[1] == 0.0);
-      switch (o.a * 4 + o.b) {
-#define X(A, B) case A * 4 + B: q_swap<A, B>(v, lam, scale); break;
-        QPAIRS(X)
-#undef X
-        default: break;
-      }
-      break;
-    }
     case Q_DENSE2: {
       switch (o.a * 4 + o.b) {
 #define X(A, B) case A * 4 + B: q_dense2<A, B>(v, o); break;
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(kQThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       const int ob = P.op_begin, oe = P.op_end;
       for (int o = ob; o < oe; ++o) apply_op(ops[o], v);
 #pragma unroll
-      for (int i = 0; i < NR; ++i) tile[tp ^ P.regpos[i]] = v[i];
+      for (int i = 0; i < NR; ++i) tile[tp ^ P.regpos_out[i]] = v[i];
       __syncthreads();
     }
 #pragma unroll

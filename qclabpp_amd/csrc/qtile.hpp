@@ -22,27 +22,39 @@ constexpr int kQTileBits = 12;     // tile: 2^12 amplitudes (64 KiB of complex<d
 constexpr int kQRegBits = 4;       // registers: 16 amplitudes per thread
 constexpr int kQThreads = 256;     // (1 << kQTileBits) / (1 << kQRegBits)
 
+// Ops act on a thread's 16 registers.  Inside a phase the host tracks which
+// register holds which logical amplitude (an affine relabeling over GF(2)):
+// X, CX, SWAP and other permutation gates only update that relabeling, plus
+// a diagonal table for their phases, so they cost no register moves; the
+// phase's write-back puts every register at its logical tile position.
+//
+// Pair ops update the register pairs (p, p ^ d) with p the member whose
+// bit lowbit(d) is clear ("base").  Bit p of `roles` set: base p holds the
+// |1> side of its pair, so the 2x2 is applied with rows and columns swapped.
 enum QKind : int32_t {
-  Q_DENSE1 = 1,   // 2x2 on register bit a                        m[0..3] row-major
-  Q_DIAGR = 2,    // v[i] *= m[i] for the 16 register indices     m[0..15]
-  Q_CDENSE1 = 3,  // 2x2 on register bit a where bit b == cs      m[0..3]
-  Q_CX = 4,       // exchange over bit a where bit b == cs
-  Q_SWAP = 5,     // exchange |a=1,b=0> and |a=0,b=1>, times m[0]
-  Q_DENSE2 = 6,   // 4x4 on register bits (a, b), row = 2*bit(a) + bit(b)   m[0..15]
-  Q_REAL1 = 7,    // 2x2 with real entries on bit a (RotationY)   re(m[0..3])
-  Q_XLIKE1 = 8,   // [[p, iq], [ir, s]], p q r s real (RotationX) re(m0) im(m1) im(m2) re(m3)
+  Q_PAIR = 1,     // complex 2x2, m[0..3] row-major
+  Q_DIAGR = 2,    // v[p] *= m[p] for the registers p in mask b, m[0..15]
+  Q_CPAIR = 3,    // Q_PAIR on the pairs whose base p has bit p of `act` set
+  Q_DENSE2 = 6,   // 4x4 on register bits (a, b), row = 2*bit(a) + bit(b), m[0..15]
+  Q_REALP = 7,    // real 2x2: re(m[0..3])
+  Q_ROTP = 8,     // real rotation [[c, -s], [s, c]]: c = re(m0), s = re(m2)
+  Q_XLIKEP = 9,   // [[p, iq], [ir, s]], p q r s real: re(m0) im(m1) im(m2) re(m3)
 };
-// Q_DIAGR: field b holds the mask of register indices whose factor is not 1.
 
 struct QOp {
   int32_t kind;
-  int32_t a, b, cs;
-  double m[32];   // complex coefficients, (re, im) interleaved
+  int32_t a, b;
+  int32_t d;             // pair distance (a register index mask)
+  uint32_t roles, act;   // per base register, see above
+  int32_t pad[2];
+  double m[32];          // complex coefficients, (re, im) interleaved
+  float mf[32];          // the same in single precision
 };
 
 struct QPhase {
-  uint32_t regmask;                 // tile-local bits held in registers
-  uint16_t regpos[1 << kQRegBits];  // LDS slot part of register index i (swizzled)
+  uint32_t regmask;                     // tile-local bits held in registers
+  uint16_t regpos[1 << kQRegBits];      // LDS slot part of register p at gather (swizzled)
+  uint16_t regpos_out[1 << kQRegBits];  // LDS slot part of register p at write-back
   int32_t op_begin, op_end;
 };
 

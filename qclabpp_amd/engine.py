@@ -113,11 +113,14 @@ class Program:
         h = C.c_void_p()
         check(lib().qsv_program_create(n, self.dtype, arr, cnt, fuse_k, diag_k, C.byref(h)))
         self._h = h
+        # kept here: module globals may already be gone when a Program is
+        # collected at interpreter exit
+        self._destroy = lib().qsv_program_destroy
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().qsv_program_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def run(self, psi, stream=None) -> None:
