@@ -44,6 +44,7 @@ inline int fuse_k() {
   const char* e = std::getenv("QCLAB_FUSE_K");
   if (!e || std::string(e) == "auto") return QSV_AUTO;
   if (std::string(e) == "tiled") return QSV_TILED;
+  if (std::string(e) == "jit") return QSV_TILED_JIT;
   const int k = std::atoi(e);
   return k < 0 ? 0 : (k > 5 ? 5 : k);
 }

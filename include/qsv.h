@@ -180,6 +180,13 @@ int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates,
 /* fuse_k == QSV_AUTO picks QSV_TILED from 22 qubits up (at least 1024 tiles
  * to spread over the GPU) and fuse_k = 5 below.                          */
 #define QSV_AUTO  (-2)
+/* fuse_k == QSV_TILED_JIT runs the same tiled passes as QSV_TILED, each pass
+ * compiled at program creation into a kernel of its own (hiprtc, gfx950)
+ * with the gate structure as constants: faster passes for a compile of
+ * about a second, done once per circuit structure and process (programs
+ * whose gates differ only in angles share the compiled passes).  Creation
+ * fails with QSV_EUNSUP where hiprtc cannot be loaded.                    */
+#define QSV_TILED_JIT (-3)
 typedef struct qsv_program qsv_program;
 
 typedef struct qsv_program_info {

@@ -16,6 +16,7 @@ HEADER = PKG.parent / "include" / "qsv.h"
 QSV_OK, QSV_EINVAL, QSV_EQUBIT, QSV_ENOMEM, QSV_EHIP, QSV_EDTYPE, QSV_ENODEV, QSV_EUNSUP = range(8)
 QSV_TILED = -1  # fuse_k value selecting tiled passes (include/qsv.h)
 QSV_AUTO = -2   # fuse_k value letting the engine choose
+QSV_TILED_JIT = -3  # tiled passes compiled per circuit structure
 QSV_C64, QSV_C128 = 1, 2
 
 

@@ -111,8 +111,9 @@ int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, i
   if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
   if (n < 1 || n > 40) return fail(QSV_EINVAL, "number of qubits must be in [1, 40]");
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
-  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO)
-    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED or QSV_AUTO");
+  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
+      fuse_k != QSV_TILED_JIT)
+    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
   if (diag_k < 0 || diag_k > kMaxDiag) return fail(QSV_EINVAL, "diag_k must be in [0, 10]");
   auto* prog = new (std::nothrow) qsv_program;
   if (!prog) return fail(QSV_ENOMEM, "host allocation failed");
@@ -204,8 +205,9 @@ int qsv_dev_simulate(void* psi, int n, int dtype, const qsv_gate* gates, size_t 
                      int fuse_k, void* stream) {
   QSV_TRY(check_state_args(psi, n, dtype));
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
-  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO)
-    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED or QSV_AUTO");
+  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
+      fuse_k != QSV_TILED_JIT)
+    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
   Program prog;
   QSV_TRY(prog.build(n, dtype, gates, ngates, fuse_k, 0));
   auto st = static_cast<hipStream_t>(stream);
@@ -218,8 +220,9 @@ int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_
                       int fuse_k) {
   QSV_TRY(check_state_args(host, n, dtype));
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
-  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO)
-    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED or QSV_AUTO");
+  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
+      fuse_k != QSV_TILED_JIT)
+    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
   Program prog;
   QSV_TRY(prog.build(n, dtype, gates, ngates, fuse_k, 0));
   const size_t bytes = (size_t(1) << n) * elem_bytes(dtype);

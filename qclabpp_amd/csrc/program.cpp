@@ -72,9 +72,11 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
   std::vector<KOp> lowered(ngates);
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, lowered[i]));
   if (fuse_k == QSV_AUTO) fuse_k = n >= kQTileBits + 10 ? QSV_TILED : kMaxFuse;
-  if (fuse_k == QSV_TILED && n < kQTileBits) fuse_k = kMaxFuse;  // one tile would be the whole state
-  if (fuse_k == QSV_TILED) {
+  if ((fuse_k == QSV_TILED || fuse_k == QSV_TILED_JIT) && n < kQTileBits)
+    fuse_k = kMaxFuse;  // one tile would be the whole state
+  if (fuse_k == QSV_TILED || fuse_k == QSV_TILED_JIT) {
     tiled = true;
+    jit = fuse_k == QSV_TILED_JIT;
     QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt));
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     qt_pass_off = 0;
@@ -84,6 +86,7 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
     std::memcpy(image.data() + qt_pass_off, qt.passes.data(), qt.passes.size() * sizeof(QPass));
     std::memcpy(image.data() + qt_phase_off, qt.phases.data(), qt.phases.size() * sizeof(QPhase));
     std::memcpy(image.data() + qt_op_off, qt.ops.data(), qt.ops.size() * sizeof(QOp));
+    if (jit) QSV_TRY(qtile_jit_build(qt, dtype == QSV_C128, qj));
     return QSV_OK;
   }
   if (fuse_k >= 1) {
@@ -158,6 +161,11 @@ int Program::run(void* psi, hipStream_t st) const {
     a.psi = psi;
     a.phases = reinterpret_cast<const QPhase*>(base + qt_phase_off);
     a.ops = reinterpret_cast<const QOp*>(base + qt_op_off);
+    if (jit) {
+      for (size_t i = 0; i < qt.passes.size(); ++i)
+        QSV_TRY(qtile_jit_launch(qj, i, psi, a.ops, st));
+      return QSV_OK;
+    }
     for (size_t i = 0; i < qt.passes.size(); ++i) {
       a.pass = passes + i;
       a.ntiles = qt.passes[i].ntiles;

@@ -8,6 +8,7 @@
 #include "fused.hpp"
 #include "fusion.hpp"
 #include "qtile.hpp"
+#include "qtile_jit.hpp"
 
 struct qsv_program_info;
 
@@ -26,7 +27,9 @@ struct Program {
   // tiled execution (fuse_k == QSV_TILED): passes, phases and ops; their
   // device copies sit in dbuf at the offsets below
   bool tiled = false;
+  bool jit = false;  // QSV_TILED_JIT: the passes run as compiled kernels
   QProgram qt;
+  QJit qj;
   size_t qt_pass_off = 0, qt_phase_off = 0, qt_op_off = 0;
   mutable void* dbuf = nullptr;
   mutable bool uploaded = false;

@@ -1,0 +1,402 @@
+// Source generation, hiprtc compilation and launch of specialised tiled
+// passes.  The generated kernels follow the tiled pass kernel (qtile.hip):
+// the same tile walk, prefetch, phase gather and write-back, with every
+// descriptor field a literal.
+#include "qtile_jit.hpp"
+
+#include <dlfcn.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <thread>
+
+namespace qsv {
+
+int device_cus();  // qtile.hip
+
+namespace {
+
+constexpr int NR = 1 << kQRegBits;
+constexpr int kJitBlocksPerCU = 32;  // as the interpreted pass kernel
+
+static_assert(sizeof(QOp) % sizeof(double) == 0 && offsetof(QOp, m) % sizeof(double) == 0 &&
+                  offsetof(QOp, mf) % sizeof(float) == 0,
+              "coefficients are addressed as arrays of the state's real type");
+
+// Device prelude of every generated module: complex helpers and the pair
+// updates, in the forms of the interpreted kernel's ops.
+const char* kPrelude = R"QSV(
+#define QC __attribute__((address_space(4)))
+typedef REAL R;
+typedef R RV __attribute__((ext_vector_type(2)));
+struct __attribute__((aligned(2 * sizeof(R)))) C { R x, y; };
+__device__ __forceinline__ C mk(R a, R b) { C c; c.x = a; c.y = b; return c; }
+__device__ __forceinline__ C cmul(C a, C b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ C cmad(C a, C b, C c) {
+  return mk(c.x + a.x * b.x - a.y * b.y, c.y + a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ C ldnt(const C* p) {
+  const RV v = __builtin_nontemporal_load(reinterpret_cast<const RV*>(p));
+  return mk(v.x, v.y);
+}
+__device__ __forceinline__ void stnt(C* p, C v) {
+  const RV w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<RV*>(p));
+}
+__device__ __forceinline__ unsigned swz(unsigned p) { return p ^ (((p >> 4) ^ (p >> 8)) & 15u); }
+__device__ __forceinline__ C cf2(const QC R* cf, int o) { return mk(cf[o], cf[o + 1]); }
+// 2x2 [[m0, m1], [m2, m3]] on (a, b)
+__device__ __forceinline__ void u_pair(C& a, C& b, C m0, C m1, C m2, C m3) {
+  const C x = a, y = b;
+  a = cmad(m1, y, cmul(m0, x));
+  b = cmad(m3, y, cmul(m2, x));
+}
+__device__ __forceinline__ void u_real(C& a, C& b, R m0, R m1, R m2, R m3) {
+  const C x = a, y = b;
+  a = mk(m0 * x.x + m1 * y.x, m0 * x.y + m1 * y.y);
+  b = mk(m2 * x.x + m3 * y.x, m2 * x.y + m3 * y.y);
+}
+// [[c, -s], [s, c]]
+__device__ __forceinline__ void u_rot(C& a, C& b, R c, R s) {
+  const C x = a, y = b;
+  a = mk(c * x.x - s * y.x, c * x.y - s * y.y);
+  b = mk(s * x.x + c * y.x, s * x.y + c * y.y);
+}
+// [[p, iq], [ir, s]]
+__device__ __forceinline__ void u_xl(C& a, C& b, R p, R q, R r, R s) {
+  const C x = a, y = b;
+  a = mk(p * x.x - q * y.y, p * x.y + q * y.x);
+  b = mk(s * y.x - r * x.y, s * y.y + r * x.x);
+}
+// 4x4 on (a0, a1, a2, a3), coefficients row-major from cf[o]
+__device__ __forceinline__ void u_d2(C& a0, C& a1, C& a2, C& a3, const QC R* cf, int o) {
+  const C x[4] = {a0, a1, a2, a3};
+  C y[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    C acc = cmul(cf2(cf, o + 8 * r), x[0]);
+#pragma unroll
+    for (int c = 1; c < 4; ++c) acc = cmad(cf2(cf, o + 8 * r + 2 * c), x[c], acc);
+    y[r] = acc;
+  }
+  a0 = y[0]; a1 = y[1]; a2 = y[2]; a3 = y[3];
+}
+)QSV";
+
+void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128) {
+  // index of coefficient slot 0 of op k in the QOp array read as R[]
+  const size_t stride = c128 ? sizeof(QOp) / sizeof(double) : sizeof(QOp) / sizeof(float);
+  const size_t base = k * stride + (c128 ? offsetof(QOp, m) / sizeof(double) : offsetof(QOp, mf) / sizeof(float));
+  auto at = [&](int i) { return std::to_string(base + i); };
+  const int K = o.d & -o.d;
+  auto pairs = [&](auto body) {
+    for (int p = 0; p < NR; ++p)
+      if (!(p & K)) body(p, p ^ o.d, bool(o.roles >> p & 1), bool(o.act >> p & 1));
+  };
+  auto v = [](int p) { return "v[" + std::to_string(p) + "]"; };
+  s << "    {\n";
+  switch (o.kind) {
+    case Q_DIAGR:
+      for (int p = 0; p < NR; ++p)
+        if (o.b >> p & 1) s << "      " << v(p) << " = cmul(cf2(cf, " << at(2 * p) << "), " << v(p) << ");\n";
+      break;
+    case Q_ROTP:
+      s << "      const R c = cf[" << at(0) << "], sn = cf[" << at(4) << "];\n";
+      pairs([&](int p, int q, bool sw, bool) {
+        s << "      u_rot(" << v(p) << ", " << v(q) << ", c, " << (sw ? "-sn" : "sn") << ");\n";
+      });
+      break;
+    case Q_REALP:
+      s << "      const R m0 = cf[" << at(0) << "], m1 = cf[" << at(2) << "], m2 = cf[" << at(4)
+        << "], m3 = cf[" << at(6) << "];\n";
+      pairs([&](int p, int q, bool sw, bool) {
+        s << "      u_real(" << v(p) << ", " << v(q) << (sw ? ", m3, m2, m1, m0);\n" : ", m0, m1, m2, m3);\n");
+      });
+      break;
+    case Q_XLIKEP:
+      s << "      const R xp = cf[" << at(0) << "], xq = cf[" << at(3) << "], xr = cf[" << at(5)
+        << "], xs = cf[" << at(6) << "];\n";
+      pairs([&](int p, int q, bool sw, bool) {
+        s << "      u_xl(" << v(p) << ", " << v(q) << (sw ? ", xs, xr, xq, xp);\n" : ", xp, xq, xr, xs);\n");
+      });
+      break;
+    case Q_PAIR:
+    case Q_CPAIR:
+      s << "      const C m0 = cf2(cf, " << at(0) << "), m1 = cf2(cf, " << at(2) << "), m2 = cf2(cf, "
+        << at(4) << "), m3 = cf2(cf, " << at(6) << ");\n";
+      pairs([&](int p, int q, bool sw, bool on) {
+        if (o.kind == Q_CPAIR && !on) return;
+        s << "      u_pair(" << v(p) << ", " << v(q) << (sw ? ", m3, m2, m1, m0);\n" : ", m0, m1, m2, m3);\n");
+      });
+      break;
+    case Q_DENSE2: {
+      const int A = 1 << o.a, B = 1 << o.b;
+      for (int i = 0; i < NR; ++i)
+        if (!(i & A) && !(i & B))
+          s << "      u_d2(" << v(i) << ", " << v(i | B) << ", " << v(i | A) << ", " << v(i | A | B)
+            << ", cf, " << at(0) << ");\n";
+      break;
+    }
+    default:
+      break;
+  }
+  s << "    }\n";
+}
+
+void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) {
+  const QPass& P = qp.passes[pi];
+  s << "extern \"C\" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))\n"
+    << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n"
+    << "  __shared__ C tile[" << (1 << kQTileBits) << "];\n"
+    << "  const unsigned tid = threadIdx.x;\n"
+    << "  unsigned long long lo = 0;\n";
+  for (int i = 0; i < 8; ++i)
+    s << "  lo |= (unsigned long long)((tid >> " << i << ") & 1u) << " << int(P.tbits[i]) << ";\n";
+  // global offsets of the thread's 16 tile positions p = tid + 256*j
+  s << "  const unsigned long long hb[16] = {";
+  for (int j = 0; j < NR; ++j) {
+    uint64_t h = 0;
+    for (int i = 0; i < 4; ++i)
+      if (j >> i & 1) h |= uint64_t(1) << P.tbits[8 + i];
+    s << (j ? ", " : "") << h << "ull";
+  }
+  s << "};\n"
+    << "  auto tile_base = [&](unsigned long long t) {\n"
+    << "    unsigned long long b = 0;\n";
+  for (int r = 0, sh = 0; r < P.nruns; ++r) {
+    s << "    b |= ((t >> " << sh << ") & " << ((uint64_t(1) << P.rlen[r]) - 1) << "ull) << " << int(P.rdst[r])
+      << ";\n";
+    sh += P.rlen[r];
+  }
+  s << "    return b;\n"
+    << "  };\n"
+    << "  const unsigned tslot = swz(tid);\n"
+    << "  unsigned long long t = blockIdx.x;\n"
+    << "  C pre[16];\n"
+    << "  if (t < ntiles) {\n"
+    << "    const unsigned long long b0 = tile_base(t);\n"
+    << "#pragma unroll\n"
+    << "    for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[b0 + (lo | hb[j])]);\n"
+    << "  }\n"
+    << "  for (; t < ntiles; t += gridDim.x) {\n"
+    << "    const unsigned long long base = tile_base(t);\n"
+    << "#pragma unroll\n"
+    << "    for (int j = 0; j < 16; ++j) tile[tslot ^ swz(256u * j)] = pre[j];\n"
+    << "    __syncthreads();\n"
+    << "    const unsigned long long tn = t + gridDim.x;\n"
+    << "    if (tn < ntiles) {\n"
+    << "      const unsigned long long bn = tile_base(tn);\n"
+    << "#pragma unroll\n"
+    << "      for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[bn + (lo | hb[j])]);\n"
+    << "    }\n";
+  for (int ph = P.phase_begin; ph < P.phase_end; ++ph) {
+    const QPhase& Q = qp.phases[ph];
+    // the coefficient pointer passes through an empty asm statement at
+    // each phase, so their loads stay in the phase that uses them instead
+    // of being hoisted out of the tile loop into (spilled) scalar registers
+    s << "    {  // phase " << ph << "\n"
+      << "      const QC R* cf = cf0;\n"
+      << "      asm volatile(\"\" : \"+s\"(cf));\n"
+      << "      unsigned x = tid;\n";
+    // spread the thread index over the tile bits outside the registers:
+    // insert a zero at each register bit, lowest first
+    for (int b = 0; b < kQTileBits; ++b)
+      if (Q.regmask >> b & 1u)
+        s << "      x = ((x >> " << b << ") << " << (b + 1) << ") | (x & " << ((1u << b) - 1) << "u);\n";
+    s << "      const unsigned tp = swz(x);\n"
+      << "      C v[16];\n";
+    for (int i = 0; i < NR; ++i) s << "      v[" << i << "] = tile[tp ^ " << Q.regpos[i] << "u];\n";
+    for (int o = Q.op_begin; o < Q.op_end; ++o) emit_op(s, qp.ops[o], size_t(o), c128);
+    for (int i = 0; i < NR; ++i) s << "      tile[tp ^ " << Q.regpos_out[i] << "u] = v[" << i << "];\n";
+    s << "      __syncthreads();\n"
+      << "    }\n";
+  }
+  s << "#pragma unroll\n"
+    << "    for (int j = 0; j < 16; ++j) stnt(&psi[base + (lo | hb[j])], tile[tslot ^ swz(256u * j)]);\n"
+    << "    __syncthreads();\n"
+    << "  }\n"
+    << "}\n";
+}
+
+// ---- hiprtc, loaded on first use ----------------------------------------
+struct Rtc {
+  bool ok = false;
+  std::string why;
+  hiprtcResult (*create)(hiprtcProgram*, const char*, const char*, int, const char**, const char**) = nullptr;
+  hiprtcResult (*compile)(hiprtcProgram, int, const char**) = nullptr;
+  hiprtcResult (*log_size)(hiprtcProgram, size_t*) = nullptr;
+  hiprtcResult (*log)(hiprtcProgram, char*) = nullptr;
+  hiprtcResult (*code_size)(hiprtcProgram, size_t*) = nullptr;
+  hiprtcResult (*code)(hiprtcProgram, char*) = nullptr;
+  hiprtcResult (*destroy)(hiprtcProgram*) = nullptr;
+};
+
+const Rtc& rtc() {
+  static Rtc r = [] {
+    Rtc x;
+    void* h = dlopen("libhiprtc.so.7", RTLD_NOW | RTLD_LOCAL);
+    if (!h) h = dlopen("/opt/rocm/lib/libhiprtc.so.7", RTLD_NOW | RTLD_LOCAL);
+    if (!h) {
+      const char* e = dlerror();
+      x.why = std::string("cannot load libhiprtc.so.7: ") + (e ? e : "");
+      return x;
+    }
+    auto sym = [&](auto& f, const char* name) {
+      f = reinterpret_cast<std::remove_reference_t<decltype(f)>>(dlsym(h, name));
+      return f != nullptr;
+    };
+    if (!sym(x.create, "hiprtcCreateProgram") || !sym(x.compile, "hiprtcCompileProgram") ||
+        !sym(x.log_size, "hiprtcGetProgramLogSize") || !sym(x.log, "hiprtcGetProgramLog") ||
+        !sym(x.code_size, "hiprtcGetCodeSize") || !sym(x.code, "hiprtcGetCode") ||
+        !sym(x.destroy, "hiprtcDestroyProgram")) {
+      x.why = "libhiprtc.so.7 lacks an expected entry point";
+      return x;
+    }
+    x.ok = true;
+    return x;
+  }();
+  return r;
+}
+
+}  // namespace
+
+struct QJitModule {
+  hipModule_t mod = nullptr;
+  ~QJitModule() {
+    if (mod) (void)hipModuleUnload(mod);
+  }
+};
+
+namespace {
+
+struct CacheKey {
+  int device;
+  std::string src;
+  bool operator<(const CacheKey& o) const { return device != o.device ? device < o.device : src < o.src; }
+};
+std::mutex g_cache_mu;
+std::map<CacheKey, std::shared_ptr<QJitModule>> g_cache;
+constexpr size_t kCacheMax = 256;  // modules kept for reuse; programs hold their own references
+
+}  // namespace
+
+std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1) {
+  std::ostringstream s;
+  s << "#define REAL " << (c128 ? "double" : "float") << "\n" << kPrelude;
+  for (size_t i = p0; i < p1 && i < qp.passes.size(); ++i) emit_pass(s, qp, i, c128);
+  return s.str();
+}
+
+int qtile_jit_compile(const std::string& src, std::vector<char>& code) {
+  const Rtc& r = rtc();
+  if (!r.ok) return fail(QSV_EUNSUP, "specialised tiled passes need hiprtc: " + r.why);
+  hiprtcProgram prog = nullptr;
+  if (r.create(&prog, src.c_str(), "qtile_pass.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+    return fail(QSV_EHIP, "hiprtcCreateProgram failed");
+  // no packed FP32 instructions, as in the rest of the device code
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult cr = r.compile(prog, int(sizeof(opts) / sizeof(opts[0])), opts);
+  if (cr != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    std::string log;
+    if (r.log_size(prog, &n) == HIPRTC_SUCCESS && n) {
+      log.resize(n);
+      r.log(prog, &log[0]);
+    }
+    r.destroy(&prog);
+    return fail(QSV_EHIP, "hiprtc compile of a tiled pass failed: " + log.substr(0, 2000));
+  }
+  size_t n = 0;
+  r.code_size(prog, &n);
+  code.resize(n);
+  r.code(prog, code.data());
+  r.destroy(&prog);
+  return n ? QSV_OK : fail(QSV_EHIP, "hiprtc returned an empty code object");
+}
+
+int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
+  out = QJit();
+  const size_t np = qp.passes.size();
+  if (!np) return QSV_OK;
+  int dev = 0;
+  QSV_CHECK_HIP(hipGetDevice(&dev));
+  // split the passes into groups of similar op counts, one module each,
+  // compiled on parallel threads
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t ng = std::min<size_t>(np, hw);
+  size_t total = 0;
+  for (const QPass& P : qp.passes)
+    total += size_t(qp.phases[P.phase_end - 1].op_end - qp.phases[P.phase_begin].op_begin) + 4;
+  std::vector<size_t> cut{0};
+  size_t acc = 0;
+  for (size_t i = 0; i < np; ++i) {
+    const QPass& P = qp.passes[i];
+    acc += size_t(qp.phases[P.phase_end - 1].op_end - qp.phases[P.phase_begin].op_begin) + 4;
+    if (acc * ng >= total * cut.size() && cut.size() < ng && i + 1 < np) cut.push_back(i + 1);
+  }
+  cut.push_back(np);
+  const size_t groups = cut.size() - 1;
+  std::vector<std::string> src(groups);
+  std::vector<std::shared_ptr<QJitModule>> mods(groups);
+  std::vector<std::vector<char>> code(groups);
+  std::vector<int> st(groups, QSV_OK);
+  std::vector<std::string> err(groups);
+  std::vector<size_t> todo;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (size_t g = 0; g < groups; ++g) {
+      src[g] = qtile_jit_source(qp, c128, cut[g], cut[g + 1]);
+      auto it = g_cache.find(CacheKey{dev, src[g]});
+      if (it != g_cache.end()) mods[g] = it->second;
+      else todo.push_back(g);
+    }
+  }
+  std::vector<std::thread> th;
+  for (size_t g : todo)
+    th.emplace_back([&, g] {
+      st[g] = qtile_jit_compile(src[g], code[g]);
+      if (st[g] != QSV_OK) err[g] = get_error();
+    });
+  for (std::thread& t : th) t.join();
+  for (size_t g : todo)
+    if (st[g] != QSV_OK) return fail(st[g], err[g]);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (size_t g : todo) {
+      auto m = std::make_shared<QJitModule>();
+      QSV_CHECK_HIP(hipModuleLoadData(&m->mod, code[g].data()));
+      mods[g] = m;
+      if (g_cache.size() >= kCacheMax) g_cache.clear();
+      g_cache[CacheKey{dev, src[g]}] = m;
+    }
+  }
+  out.mods = mods;
+  out.fn.resize(np);
+  out.ntiles.resize(np);
+  for (size_t g = 0; g < groups; ++g)
+    for (size_t i = cut[g]; i < cut[g + 1]; ++i) {
+      const std::string name = "qtp_" + std::to_string(i);
+      QSV_CHECK_HIP(hipModuleGetFunction(&out.fn[i], mods[g]->mod, name.c_str()));
+      out.ntiles[i] = qp.passes[i].ntiles;
+    }
+  return QSV_OK;
+}
+
+int qtile_jit_launch(const QJit& j, size_t i, void* psi, const QOp* ops, hipStream_t st) {
+  if (i >= j.fn.size()) return fail(QSV_EINVAL, "tiled pass index out of range");
+  uint64_t ntiles = j.ntiles[i];
+  const uint64_t want = uint64_t(device_cus()) * kJitBlocksPerCU;
+  const unsigned grid = unsigned(ntiles < want ? ntiles : want);
+  const void* cf = ops;
+  void* args[] = {&psi, &cf, &ntiles};
+  QSV_CHECK_HIP(hipModuleLaunchKernel(j.fn[i], grid, 1, 1, kQThreads, 1, 1, 0, st, args, nullptr));
+  return QSV_OK;
+}
+
+}  // namespace qsv

@@ -1,0 +1,42 @@
+// Specialised tiled passes: the passes, phases and ops of a compiled
+// QProgram are emitted as straight-line HIP source, one kernel per pass,
+// and compiled at program creation with hiprtc for the device.  The op
+// kinds, register pairs, roles and LDS slots become constants, so a pass
+// runs without the op interpreter's dispatch, selects and register copies.
+// Coefficients stay in device memory (the QOp array of the program), so
+// programs with the same structure share one compiled module.
+//
+// No counterpart in the reference (src/qgates/apply.hpp:258-371 sweeps the
+// state once per gate); the result is the product of the same lowered gates.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "qtile.hpp"
+
+namespace qsv {
+
+struct QJitModule;  // a loaded code object, shared by programs of equal structure
+
+struct QJit {
+  std::vector<std::shared_ptr<QJitModule>> mods;
+  std::vector<hipFunction_t> fn;  // per pass
+  std::vector<uint64_t> ntiles;   // per pass
+};
+
+// Source of the kernels for passes [p0, p1) of qp (test hook and build step).
+std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1);
+
+// Generate, compile (or take from the process-wide cache) and load the
+// pass kernels for the current device.
+int qtile_jit_build(const QProgram& qp, bool c128, QJit& out);
+
+// Launch pass i.  ops: the program's device QOp array.
+int qtile_jit_launch(const QJit& j, size_t i, void* psi, const QOp* ops, hipStream_t st);
+
+// Compile one source to a gfx950 code object (no device needed).
+int qtile_jit_compile(const std::string& src, std::vector<char>& code);
+
+}  // namespace qsv

@@ -99,6 +99,7 @@ def simulate_host(vec: np.ndarray, gates) -> np.ndarray:
 
 TILED = _lib.QSV_TILED
 AUTO = _lib.QSV_AUTO
+TILED_JIT = _lib.QSV_TILED_JIT
 
 
 class Program:

@@ -1,7 +1,8 @@
-"""GPU parity of tiled programs (fuse_k = QSV_TILED): passes over LDS tiles
-of 2^12 amplitudes, gates applied in registers phase by phase.  Checked
-against the oracle on mixed circuits with every gate class, both control
-states, Op N/C and every qubit, and at full size through the inverse."""
+"""GPU parity of tiled programs: passes over LDS tiles of 2^12 amplitudes,
+gates applied in registers phase by phase, run by the interpreting pass
+kernel (fuse_k = QSV_TILED) and as passes compiled per circuit structure
+(QSV_TILED_JIT).  Checked against the oracle on mixed circuits with every
+gate class, both control states, Op N/C and every qubit."""
 import numpy as np
 import pytest
 
@@ -13,6 +14,8 @@ from qclabpp_amd.rng import normal_state
 from tests.gatecases import one_qubit_gates, random_state, two_qubit_gates
 
 pytestmark = pytest.mark.gpu
+
+FORMS = [pytest.param(E.TILED, id="interp"), pytest.param(E.TILED_JIT, id="jit")]
 
 
 def run(prog, psi0):
@@ -41,40 +44,64 @@ def every_kind(n, seed):
     return [circ[i] for i in order]
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("n", [12, 13, 15, 18])
-def test_tiled_every_kind_c128(n):
-    print(f"tiled every kind n={n}", flush=True)
+def test_tiled_every_kind_c128(n, form):
+    print(f"tiled every kind n={n} form={form}", flush=True)
     circ = every_kind(n, n)
     psi = random_state(n, 5, np.complex128)
     want = oracle.apply(n, circ, psi)
-    prog = E.Program(n, circ, fuse_k=E.TILED)
+    prog = E.Program(n, circ, fuse_k=form)
     got = run(prog, psi)
     assert np.abs(got - want).max() < 1e-12
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("n", [13, 17])
-def test_tiled_every_kind_c64(n):
+def test_tiled_every_kind_c64(n, form):
     circ = every_kind(n, 2 * n)
     psi = random_state(n, 6, np.complex128)
     want = oracle.apply(n, circ, psi)
-    got = run(E.Program(n, circ, fuse_k=E.TILED, dtype="c64"), psi.astype(np.complex64))
+    got = run(E.Program(n, circ, fuse_k=form, dtype="c64"), psi.astype(np.complex64))
     assert np.abs(got - want).max() < 5e-5
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("n", [16, 20])
-def test_tiled_brickwork_and_ansatz(n):
+def test_tiled_brickwork_and_ansatz(n, form):
     circ = W.brickwork(n, 20, seed=7) + W.tfim_ansatz(n, 2)
     psi = random_state(n, 9, np.complex128)
     want = oracle.apply(n, circ, psi)
-    prog = E.Program(n, circ, fuse_k=E.TILED)
+    prog = E.Program(n, circ, fuse_k=form)
     assert prog.info()["nlaunch"] >= 2
     got = run(prog, psi)
     assert np.abs(got - want).max() < 1e-12
 
 
-def test_tiled_qft20_is_dft():
+@pytest.mark.parametrize("form", FORMS)
+def test_tiled_qft20_is_dft(form):
     n = 20
     x = normal_state(W.SEED_QFT, n)
     want = np.fft.ifft(x) * np.sqrt(1 << n)
-    got = run(E.Program(n, W.qft(n), fuse_k=E.TILED), x)
+    got = run(E.Program(n, W.qft(n), fuse_k=form), x)
     assert np.abs(got - want).max() < 1e-12
+
+
+def test_jit_structure_shared_across_angles():
+    """Programs whose gates differ only in angles share compiled passes
+    (the second creation finds them cached) and each gives its own result."""
+    import time
+    n = 16
+    a = W.brickwork(n, 8, seed=11)
+    # same structure, new angles (none near 0 or pi)
+    b = [g.with_theta(0.5 + 0.9 * abs(np.sin(3.0 * g.theta))) if g.theta else g for g in a]
+    psi = random_state(n, 4, np.complex128)
+    t0 = time.perf_counter()
+    pa = E.Program(n, a, fuse_k=E.TILED_JIT)
+    t1 = time.perf_counter()
+    pb = E.Program(n, b, fuse_k=E.TILED_JIT)
+    t2 = time.perf_counter()
+    print(f"jit create: first {t1 - t0:.2f} s, same structure {t2 - t1:.3f} s", flush=True)
+    assert np.abs(run(pa, psi) - oracle.apply(n, a, psi)).max() < 1e-12
+    assert np.abs(run(pb, psi) - oracle.apply(n, b, psi)).max() < 1e-12
+    assert t2 - t1 < 0.5 * (t1 - t0)
