@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <set>
+#include <cstdlib>
 
 namespace qsv {
 
@@ -173,12 +174,21 @@ struct Frame {
 
 }  // namespace
 
-int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out) {
+namespace {
+
+// One compile with a given rule for ties between tile candidates that
+// absorb equally many pending gates: strategy 0 takes the highest address
+// bit, 1 the lowest, others a pseudo-random one (seeded by the strategy).
+int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int strategy,
+                       QProgram& out) {
   constexpr int T = kQTileBits, RB = kQRegBits, NR = 1 << kQRegBits;
-  if (n < T) return fail(QSV_EUNSUP, "tiled execution needs at least " + std::to_string(T) + " qubits");
-  for (const KOp& op : in)
-    if (op.kind == K_DENSEK || op.kind == K_DIAGK || op.nq > 2)
-      return fail(QSV_EINVAL, "tiled execution takes unfused gates");
+  uint64_t rng = 0x9E3779B97F4A7C15ull * uint64_t(strategy + 1);
+  auto next = [&]() {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
   out = QProgram();
   out.n = n;
   std::vector<KOp> rem = merge_one_qubit_runs(in, n);
@@ -204,15 +214,18 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
         for (int j = 0; j < rem[i].nq; ++j)
           if (!inS[rem[i].q[j]]) cand.insert(rem[i].q[j]);
       if (cand.empty()) {
-        for (int q = 0; q < n && size < T; ++q) add(q);
+        // no pending gate needs another qubit: take the lowest address bits,
+        // which lengthens the contiguous runs each tile reads and writes
+        for (int q = n - 1; q >= 0 && size < T; --q) add(q);
         break;
       }
       int best = -1, bq = -1;
-      for (int q : cand) {
+      for (int q : cand) {  // ascending qubit = descending address bit
         inS[q] = 1;
         const int a = count_absorbed(rem, window, inS, blocked);
         inS[q] = 0;
-        if (a > best) {
+        const bool take = a > best || (a == best && (strategy == 1 || (strategy > 1 && (next() & 1))));
+        if (take) {
           best = a;
           bq = q;
         }
@@ -433,6 +446,28 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
     P.phase_end = int(out.phases.size());
     out.passes.push_back(P);
     rem.swap(deferred);
+  }
+  return QSV_OK;
+}
+
+}  // namespace
+
+int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out) {
+  if (n < kQTileBits)
+    return fail(QSV_EUNSUP, "tiled execution needs at least " + std::to_string(kQTileBits) + " qubits");
+  for (const KOp& op : in)
+    if (op.kind == K_DENSEK || op.kind == K_DIAGK || op.nq > 2)
+      return fail(QSV_EINVAL, "tiled execution takes unfused gates");
+  // The greedy tile choice depends on how ties are broken; a few rules are
+  // tried and the schedule with the fewest passes (then phases) is kept:
+  // each pass costs a full read and write of the state.
+  constexpr int kStrategies = 8;
+  for (int st = 0; st < kStrategies; ++st) {
+    QProgram cand;
+    QSV_TRY(compile_qtile_with(in, n, line_bits, st, cand));
+    if (st == 0 || cand.passes.size() < out.passes.size() ||
+        (cand.passes.size() == out.passes.size() && cand.phases.size() < out.phases.size()))
+      out = std::move(cand);
   }
   return QSV_OK;
 }

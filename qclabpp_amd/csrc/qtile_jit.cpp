@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -148,78 +149,152 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128) {
   s << "    }\n";
 }
 
+// 64-bit literal
+std::string u64(uint64_t x) { return std::to_string(x) + "ull"; }
+
+// Global offset (in amplitudes) of tile position p of pass P.
+uint64_t gofs(const QPass& P, uint32_t p) {
+  uint64_t o = 0;
+  for (int b = 0; b < kQTileBits; ++b)
+    if (p >> b & 1u) o |= uint64_t(1) << P.tbits[b];
+  return o;
+}
+
+// Expression for the global offset of this thread's share of a phase:
+// thread index bit j sits on the j-th lowest tile bit outside the phase's
+// registers.
+std::string thread_gofs(const QPass& P, uint32_t regmask) {
+  std::ostringstream e;
+  e << "(0ull";
+  for (int b = 0, j = 0; b < kQTileBits && j < 8; ++b)
+    if (!(regmask >> b & 1u)) {
+      e << " | ((unsigned long long)((tid >> " << j << ") & 1u) << " << int(P.tbits[b]) << ")";
+      ++j;
+    }
+  e << ")";
+  return e.str();
+}
+
+// A pass as one kernel.  Each tile goes HBM -> phases -> HBM.  A phase
+// gathers its 16 register amplitudes from the LDS tile, applies its ops and
+// writes them back; the first phase instead loads its registers straight
+// from HBM (prefetched one tile ahead) and the last phase stores straight to
+// HBM, whenever the lanes of a wave then still cover whole 128-byte lines
+// (the phase's registers leave the line bits free).  A one-phase pass with
+// both ends direct never touches LDS.
 void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) {
   const QPass& P = qp.passes[pi];
-  s << "extern \"C\" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))\n"
-    << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n"
-    << "  __shared__ C tile[" << (1 << kQTileBits) << "];\n"
-    << "  const unsigned tid = threadIdx.x;\n"
-    << "  unsigned long long lo = 0;\n";
-  for (int i = 0; i < 8; ++i)
-    s << "  lo |= (unsigned long long)((tid >> " << i << ") & 1u) << " << int(P.tbits[i]) << ";\n";
-  // global offsets of the thread's 16 tile positions p = tid + 256*j
-  s << "  const unsigned long long hb[16] = {";
-  for (int j = 0; j < NR; ++j) {
-    uint64_t h = 0;
-    for (int i = 0; i < 4; ++i)
-      if (j >> i & 1) h |= uint64_t(1) << P.tbits[8 + i];
-    s << (j ? ", " : "") << h << "ull";
-  }
-  s << "};\n"
+  const int K = P.phase_end - P.phase_begin;
+  const QPhase& F = qp.phases[P.phase_begin];
+  const QPhase& L = qp.phases[P.phase_end - 1];
+  const uint32_t line = c128 ? 7u : 15u;  // tile bits of one 128-byte line
+  const bool din = !(F.regmask & line) && !getenv("QSV_JIT_NODIRECT");
+  const bool dout = !(L.regmask & line) && !getenv("QSV_JIT_NODIRECT");
+  int wpe = c128 ? 2 : 3;
+  if (const char* e = getenv(c128 ? "QSV_JIT_WPE128" : "QSV_JIT_WPE64")) wpe = atoi(e);
+  s << "extern \"C\" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(" << wpe << ")))\n"
+    << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
+  if (!(din && dout && K == 1)) s << "  __shared__ C tile[" << (1 << kQTileBits) << "];\n";
+  s << "  const unsigned tid = threadIdx.x;\n"
     << "  auto tile_base = [&](unsigned long long t) {\n"
     << "    unsigned long long b = 0;\n";
   for (int r = 0, sh = 0; r < P.nruns; ++r) {
-    s << "    b |= ((t >> " << sh << ") & " << ((uint64_t(1) << P.rlen[r]) - 1) << "ull) << " << int(P.rdst[r])
+    s << "    b |= ((t >> " << sh << ") & " << u64((uint64_t(1) << P.rlen[r]) - 1) << ") << " << int(P.rdst[r])
       << ";\n";
     sh += P.rlen[r];
   }
   s << "    return b;\n"
-    << "  };\n"
-    << "  const unsigned tslot = swz(tid);\n"
+    << "  };\n";
+  // offsets of the 16 amplitudes a thread loads (and stores) at the ends:
+  // the first (last) phase's registers, or tile positions tid + 256*j
+  std::vector<uint64_t> in_ofs(NR), out_ofs(NR);
+  for (int i = 0; i < NR; ++i) {
+    in_ofs[i] = din ? gofs(P, qtile_swizzle(F.regpos[i])) : gofs(P, uint32_t(kQThreads * i));
+    out_ofs[i] = dout ? gofs(P, qtile_swizzle(L.regpos_out[i])) : gofs(P, uint32_t(kQThreads * i));
+  }
+  s << "  const unsigned long long gin = " << (din ? thread_gofs(P, F.regmask) : thread_gofs(P, 0xF00u)) << ";\n"
+    << "  const unsigned long long gout = " << (dout ? thread_gofs(P, L.regmask) : thread_gofs(P, 0xF00u)) << ";\n";
+  auto arr = [&](const char* name, const std::vector<uint64_t>& v) {
+    s << "  const unsigned long long " << name << "[16] = {";
+    for (int i = 0; i < NR; ++i) s << (i ? ", " : "") << u64(v[i]);
+    s << "};\n";
+  };
+  arr("oin", in_ofs);
+  arr("oout", out_ofs);
+  s << "  const unsigned tslot = swz(tid);\n"
     << "  unsigned long long t = blockIdx.x;\n"
     << "  C pre[16];\n"
     << "  if (t < ntiles) {\n"
-    << "    const unsigned long long b0 = tile_base(t);\n"
+    << "    const unsigned long long b0 = tile_base(t) + gin;\n"
     << "#pragma unroll\n"
-    << "    for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[b0 + (lo | hb[j])]);\n"
+    << "    for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[b0 + oin[j]]);\n"
     << "  }\n"
     << "  for (; t < ntiles; t += gridDim.x) {\n"
     << "    const unsigned long long base = tile_base(t);\n"
+    << "    C v[16];\n";
+  if (din) {
+    s << "#pragma unroll\n"
+      << "    for (int j = 0; j < 16; ++j) v[j] = pre[j];\n";
+  } else {
+    s << "#pragma unroll\n"
+      << "    for (int j = 0; j < 16; ++j) tile[tslot ^ swz(256u * j)] = pre[j];\n"
+      << "    __syncthreads();\n";
+  }
+  s << "    {\n"
+    << "      const unsigned long long tn = t + gridDim.x;\n"
+    << "      if (tn < ntiles) {\n"
+    << "        const unsigned long long bn = tile_base(tn) + gin;\n"
     << "#pragma unroll\n"
-    << "    for (int j = 0; j < 16; ++j) tile[tslot ^ swz(256u * j)] = pre[j];\n"
-    << "    __syncthreads();\n"
-    << "    const unsigned long long tn = t + gridDim.x;\n"
-    << "    if (tn < ntiles) {\n"
-    << "      const unsigned long long bn = tile_base(tn);\n"
-    << "#pragma unroll\n"
-    << "      for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[bn + (lo | hb[j])]);\n"
+    << "        for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[bn + oin[j]]);\n"
+    << "      }\n"
     << "    }\n";
-  for (int ph = P.phase_begin; ph < P.phase_end; ++ph) {
+  for (int k = 0; k < K; ++k) {
+    const int ph = P.phase_begin + k;
     const QPhase& Q = qp.phases[ph];
-    // the coefficient pointer passes through an empty asm statement at
-    // each phase, so their loads stay in the phase that uses them instead
-    // of being hoisted out of the tile loop into (spilled) scalar registers
+    const bool gather = !(k == 0 && din);
+    const bool scatter = !(k == K - 1 && dout);
+    // the coefficient pointer and the thread index pass through empty asm
+    // statements at each phase, so coefficient loads and LDS addresses are
+    // formed in the phase that uses them instead of being hoisted out of
+    // the tile loop into registers held (and spilled) across the pass
     s << "    {  // phase " << ph << "\n"
       << "      const QC R* cf = cf0;\n"
-      << "      asm volatile(\"\" : \"+s\"(cf));\n"
-      << "      unsigned x = tid;\n";
-    // spread the thread index over the tile bits outside the registers:
-    // insert a zero at each register bit, lowest first
-    for (int b = 0; b < kQTileBits; ++b)
-      if (Q.regmask >> b & 1u)
-        s << "      x = ((x >> " << b << ") << " << (b + 1) << ") | (x & " << ((1u << b) - 1) << "u);\n";
-    s << "      const unsigned tp = swz(x);\n"
-      << "      C v[16];\n";
-    for (int i = 0; i < NR; ++i) s << "      v[" << i << "] = tile[tp ^ " << Q.regpos[i] << "u];\n";
+      << "      asm volatile(\"\" : \"+s\"(cf));\n";
+    if (gather || scatter) {
+      s << "      unsigned x = tid;\n"
+        << "      asm volatile(\"\" : \"+v\"(x));\n";
+      // spread the thread index over the tile bits outside the registers:
+      // insert a zero at each register bit, lowest first
+      for (int b = 0; b < kQTileBits; ++b)
+        if (Q.regmask >> b & 1u)
+          s << "      x = ((x >> " << b << ") << " << (b + 1) << ") | (x & " << ((1u << b) - 1) << "u);\n";
+      s << "      const unsigned tp = swz(x);\n";
+    }
+    if (gather) {
+      for (int i = 0; i < NR; ++i) s << "      v[" << i << "] = tile[tp ^ " << Q.regpos[i] << "u];\n";
+      // the LDS tile is written next by the following tile's first phase
+      if (!scatter) s << "      __syncthreads();\n";
+    }
     for (int o = Q.op_begin; o < Q.op_end; ++o) emit_op(s, qp.ops[o], size_t(o), c128);
-    for (int i = 0; i < NR; ++i) s << "      tile[tp ^ " << Q.regpos_out[i] << "u] = v[" << i << "];\n";
-    s << "      __syncthreads();\n"
+    if (scatter) {
+      for (int i = 0; i < NR; ++i) s << "      tile[tp ^ " << Q.regpos_out[i] << "u] = v[" << i << "];\n";
+      s << "      __syncthreads();\n";
+    } else {
+      s << "      const unsigned long long bo = base + gout;\n"
+        << "#pragma unroll\n"
+        << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], v[j]);\n";
+    }
+    s << "    }\n";
+  }
+  if (!dout) {
+    s << "    {\n"
+      << "      const unsigned long long bo = base + gout;\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], tile[tslot ^ swz(256u * j)]);\n"
+      << "      __syncthreads();\n"
       << "    }\n";
   }
-  s << "#pragma unroll\n"
-    << "    for (int j = 0; j < 16; ++j) stnt(&psi[base + (lo | hb[j])], tile[tslot ^ swz(256u * j)]);\n"
-    << "    __syncthreads();\n"
-    << "  }\n"
+  s << "  }\n"
     << "}\n";
 }
 
