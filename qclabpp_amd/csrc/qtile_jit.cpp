@@ -10,7 +10,6 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdio>
-#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -188,10 +187,11 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
   const QPhase& F = qp.phases[P.phase_begin];
   const QPhase& L = qp.phases[P.phase_end - 1];
   const uint32_t line = c128 ? 7u : 15u;  // tile bits of one 128-byte line
-  const bool din = !(F.regmask & line) && !getenv("QSV_JIT_NODIRECT");
-  const bool dout = !(L.regmask & line) && !getenv("QSV_JIT_NODIRECT");
-  int wpe = c128 ? 2 : 3;
-  if (const char* e = getenv(c128 ? "QSV_JIT_WPE128" : "QSV_JIT_WPE64")) wpe = atoi(e);
+  const bool din = !(F.regmask & line);
+  const bool dout = !(L.regmask & line);
+  // waves per SIMD the register budget is sized for: a complex128 tile
+  // (64 KiB of LDS) allows two blocks per CU, a complex64 one three
+  const int wpe = c128 ? 2 : 3;
   s << "extern \"C\" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(" << wpe << ")))\n"
     << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
   if (!(din && dout && K == 1)) s << "  __shared__ C tile[" << (1 << kQTileBits) << "];\n";
