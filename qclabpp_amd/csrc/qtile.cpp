@@ -144,6 +144,88 @@ std::vector<Group> make_phases(const std::vector<KOp>& ops, int limit) {
   return phases;
 }
 
+// Greedy phase grouping, the pass chooser at register scale: a phase's
+// qubits are the first pending op's, then one by one the qubit that lets the
+// phase absorb the most pending ops (ties: lowest qubit, highest, or
+// pseudo-random by `strategy`); it absorbs every op on its qubits that
+// depends on no op left out.
+std::vector<Group> make_phases_greedy(const std::vector<KOp>& ops, int nq, int limit, int strategy) {
+  std::vector<int> rem(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) rem[i] = int(i);
+  std::vector<Group> out;
+  std::vector<char> inS(nq), blocked(nq);
+  uint64_t rng = 0xD1B54A32D192ED03ull * uint64_t(strategy + 1);
+  auto absorbed = [&](std::vector<int>* take) {
+    std::fill(blocked.begin(), blocked.end(), 0);
+    int cnt = 0;
+    for (int i : rem) {
+      const KOp& op = ops[i];
+      bool ok = true;
+      for (int j = 0; j < op.nq; ++j) ok = ok && inS[op.q[j]] && !blocked[op.q[j]];
+      if (ok) {
+        ++cnt;
+        if (take) take->push_back(i);
+      } else {
+        for (int j = 0; j < op.nq; ++j) blocked[op.q[j]] = 1;
+      }
+    }
+    return cnt;
+  };
+  while (!rem.empty()) {
+    std::fill(inS.begin(), inS.end(), 0);
+    int size = 0;
+    const KOp& first = ops[rem[0]];
+    for (int j = 0; j < first.nq; ++j)
+      if (!inS[first.q[j]]) {
+        inS[first.q[j]] = 1;
+        ++size;
+      }
+    while (size < limit) {
+      std::vector<char> cand(nq, 0);
+      bool any = false;
+      for (int i : rem)
+        for (int j = 0; j < ops[i].nq; ++j)
+          if (!inS[ops[i].q[j]]) cand[ops[i].q[j]] = 1, any = true;
+      if (!any) break;
+      int best = -1, bq = -1;
+      for (int q = 0; q < nq; ++q) {
+        if (!cand[q]) continue;
+        inS[q] = 1;
+        const int a = absorbed(nullptr);
+        inS[q] = 0;
+        bool take = a > best;
+        if (a == best) {
+          if (strategy == 1) take = true;
+          else if (strategy > 1) {
+            rng ^= rng << 13;
+            rng ^= rng >> 7;
+            rng ^= rng << 17;
+            take = rng & 1;
+          }
+        }
+        if (take) {
+          best = a;
+          bq = q;
+        }
+      }
+      inS[bq] = 1;
+      ++size;
+    }
+    Group g;
+    for (int q = 0; q < nq; ++q)
+      if (inS[q]) g.q.push_back(q);
+    absorbed(&g.ops);
+    std::vector<char> gone(ops.size(), 0);
+    for (int i : g.ops) gone[i] = 1;
+    std::vector<int> next;
+    for (int i : rem)
+      if (!gone[i]) next.push_back(i);
+    rem.swap(next);
+    out.push_back(std::move(g));
+  }
+  return out;
+}
+
 void set_c(QOp& o, int i, cd z) {
   o.m[2 * i] = z.real();
   o.m[2 * i + 1] = z.imag();
@@ -274,7 +356,14 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int str
     };
     // ---- phases
     P.phase_begin = int(out.phases.size());
-    for (const Group& g : make_phases(mine, RB)) {
+    // phase grouping: the fusion-style one or the greedy one under a few tie
+    // rules, whichever gives the fewest phases
+    std::vector<Group> groups = make_phases(mine, RB);
+    for (int st = 0; st < 4; ++st) {
+      std::vector<Group> alt = make_phases_greedy(mine, n, RB, st);
+      if (alt.size() < groups.size()) groups.swap(alt);
+    }
+    for (const Group& g : groups) {
       uint32_t regmask = 0;
       for (int q : g.q) regmask |= 1u << local(q);
       // pad with the highest free tile bits: the low tile bits then index
