@@ -3,6 +3,7 @@
 #include <cstring>
 #include <new>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -57,6 +58,96 @@ int reserve_cache(size_t bytes) {
     return fail(QSV_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
   }
   g_cache.bytes = bytes;
+  return QSV_OK;
+}
+
+// Transfers between a pageable host vector and the device state go
+// through two pinned staging buffers: the host-side copy of one chunk (split
+// over several threads) overlaps the DMA of the other.  A plain hipMemcpy
+// from pageable memory reached 22 GB/s for the two transfers of a 16 GiB
+// state.  Guarded by g_cache.mu.
+constexpr size_t kStageBytes = size_t(64) << 20;
+constexpr int kStageThreads = 8;
+
+struct Stage {
+  void* buf[2] = {nullptr, nullptr};
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+} g_stage;
+
+int stage_init() {
+  if (g_stage.st) return QSV_OK;
+  for (int b = 0; b < 2; ++b) {
+    if (hipHostMalloc(&g_stage.buf[b], kStageBytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(QSV_ENOMEM, "pinned staging allocation failed");
+    }
+    QSV_CHECK_HIP(hipEventCreateWithFlags(&g_stage.ev[b], hipEventDisableTiming));
+  }
+  QSV_CHECK_HIP(hipStreamCreateWithFlags(&g_stage.st, hipStreamNonBlocking));
+  return QSV_OK;
+}
+
+int stage_release() {
+  for (int b = 0; b < 2; ++b) {
+    if (g_stage.buf[b]) QSV_CHECK_HIP(hipHostFree(g_stage.buf[b]));
+    if (g_stage.ev[b]) QSV_CHECK_HIP(hipEventDestroy(g_stage.ev[b]));
+    g_stage.buf[b] = nullptr;
+    g_stage.ev[b] = nullptr;
+  }
+  if (g_stage.st) QSV_CHECK_HIP(hipStreamDestroy(g_stage.st));
+  g_stage.st = nullptr;
+  return QSV_OK;
+}
+
+// memcpy split over threads (page-aligned slices)
+void par_copy(void* dst, const void* src, size_t bytes) {
+  const int nt = bytes >= (size_t(8) << 20) ? kStageThreads : 1;
+  if (nt == 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t slice = (bytes / nt + 4095) & ~size_t(4095);
+  std::thread th[kStageThreads];
+  for (int t = 0; t < nt; ++t) {
+    const size_t a = std::min(bytes, size_t(t) * slice), b = std::min(bytes, a + slice);
+    th[t] = std::thread([=] { std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a); });
+  }
+  for (int t = 0; t < nt; ++t) th[t].join();
+}
+
+int upload(void* dev, const void* host, size_t bytes) {
+  QSV_TRY(stage_init());
+  for (size_t off = 0, k = 0; off < bytes; off += kStageBytes, ++k) {
+    const size_t len = std::min(kStageBytes, bytes - off);
+    const int b = int(k & 1);
+    QSV_CHECK_HIP(hipEventSynchronize(g_stage.ev[b]));  // buffer b's previous DMA is done
+    par_copy(g_stage.buf[b], static_cast<const char*>(host) + off, len);
+    QSV_CHECK_HIP(hipMemcpyAsync(static_cast<char*>(dev) + off, g_stage.buf[b], len,
+                                 hipMemcpyHostToDevice, g_stage.st));
+    QSV_CHECK_HIP(hipEventRecord(g_stage.ev[b], g_stage.st));
+  }
+  QSV_CHECK_HIP(hipStreamSynchronize(g_stage.st));
+  return QSV_OK;
+}
+
+int download(void* host, const void* dev, size_t bytes) {
+  QSV_TRY(stage_init());
+  const size_t nk = (bytes + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](size_t k) -> int {
+    const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    QSV_CHECK_HIP(hipMemcpyAsync(g_stage.buf[k & 1], static_cast<const char*>(dev) + off, len,
+                                 hipMemcpyDeviceToHost, g_stage.st));
+    QSV_CHECK_HIP(hipEventRecord(g_stage.ev[k & 1], g_stage.st));
+    return QSV_OK;
+  };
+  for (size_t k = 0; k < nk && k < 2; ++k) QSV_TRY(issue(k));
+  for (size_t k = 0; k < nk; ++k) {
+    const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    QSV_CHECK_HIP(hipEventSynchronize(g_stage.ev[k & 1]));
+    par_copy(static_cast<char*>(host) + off, g_stage.buf[k & 1], len);
+    if (k + 2 < nk) QSV_TRY(issue(k + 2));
+  }
   return QSV_OK;
 }
 
@@ -195,9 +286,10 @@ int qsv_host_apply_gates(void* host, int n, int dtype, const qsv_gate* gates, si
   const size_t bytes = (size_t(1) << n) * elem_bytes(dtype);
   std::lock_guard<std::mutex> lk(g_cache.mu);
   QSV_TRY(reserve_cache(bytes));
-  QSV_CHECK_HIP(hipMemcpy(g_cache.ptr, host, bytes, hipMemcpyHostToDevice));
+  QSV_TRY(upload(g_cache.ptr, host, bytes));
   for (const KOp& op : ops) QSV_TRY(apply_lowered(op, g_cache.ptr, n, dtype, nullptr));
-  QSV_CHECK_HIP(hipMemcpy(host, g_cache.ptr, bytes, hipMemcpyDeviceToHost));
+  QSV_CHECK_HIP(hipStreamSynchronize(nullptr));
+  QSV_TRY(download(host, g_cache.ptr, bytes));
   return QSV_OK;
 }
 
@@ -228,9 +320,10 @@ int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_
   const size_t bytes = (size_t(1) << n) * elem_bytes(dtype);
   std::lock_guard<std::mutex> lk(g_cache.mu);
   QSV_TRY(reserve_cache(bytes));
-  QSV_CHECK_HIP(hipMemcpy(g_cache.ptr, host, bytes, hipMemcpyHostToDevice));
+  QSV_TRY(upload(g_cache.ptr, host, bytes));
   QSV_TRY(prog.run(g_cache.ptr, nullptr));
-  QSV_CHECK_HIP(hipMemcpy(host, g_cache.ptr, bytes, hipMemcpyDeviceToHost));
+  QSV_CHECK_HIP(hipStreamSynchronize(nullptr));
+  QSV_TRY(download(host, g_cache.ptr, bytes));
   return QSV_OK;
 }
 
@@ -239,6 +332,7 @@ int qsv_release_cache(void) {
   if (g_cache.ptr) QSV_CHECK_HIP(hipFree(g_cache.ptr));
   g_cache.ptr = nullptr;
   g_cache.bytes = 0;
+  QSV_TRY(stage_release());
   return QSV_OK;
 }
 

@@ -104,3 +104,26 @@ def test_host_vector_entry():
     got = E.simulate_host(psi, circ)
     want = oracle.apply(n, circ, psi)
     assert np.abs(got - want).max() < 1e-13
+
+
+@pytest.mark.parametrize("n,dt", [(24, np.complex128), (25, np.complex64), (13, np.complex64)])
+def test_host_entry_staged_transfers(n, dt):
+    """The std::vector entry moves the state through pinned double buffers of
+    64 MiB: several chunks (n=24 complex128: 4; n=25 complex64: 4) and one
+    partial chunk (n=13).  Same result as the device-resident program."""
+    import ctypes as C
+
+    import torch
+
+    from qclabpp_amd import _lib
+    from qclabpp_amd import gates as G
+    circ = W.brickwork(n, 3, seed=5)
+    x = random_state(n, 7, np.complex128).astype(dt)
+    h = x.copy()
+    arr, cnt = G.records(circ)
+    code = _lib.QSV_C128 if dt == np.complex128 else _lib.QSV_C64
+    _lib.check(_lib.lib().qsv_host_simulate(h.ctypes.data_as(C.c_void_p), n, code, arr, cnt, 5))
+    d = torch.from_numpy(x.copy()).cuda()
+    E.Program(n, circ, fuse_k=5, dtype="c128" if dt == np.complex128 else "c64").run(d)
+    torch.cuda.synchronize()
+    assert np.array_equal(h, d.cpu().numpy())
