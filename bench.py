@@ -56,6 +56,8 @@ def parse():
                     help="qubits of the complex64-vs-complex128 comparison (0 skips it)")
     ap.add_argument("--precision-big-n", type=int, default=33,
                     help="qubits of the complex64-only run (64 GiB at 33; 0 skips it)")
+    ap.add_argument("--host-entry", type=int, default=1,
+                    help="time the std::vector entry (PCIe in and out) on the main workload (0 skips it)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the dominant kernels and a 16 GiB copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -184,6 +186,36 @@ def adjoint_big_leg(args):
             "state_bytes": (1 << n) * 16, "energy": e_adj, "adjoint_s": t_adj,
             "fd_params": which, "fd_s_for_those": t_fd,
             "max_abs_adjoint_minus_fd": float(np.abs(g_adj[which] - g_fd).max())}
+
+
+def host_entry_leg(args, circ, ms_jit):
+    """qsv_host_simulate (the std::vector entry) on a 16 GiB numpy vector:
+    upload through the pinned staging buffers, the compiled tiled passes
+    (cached from the main leg, so no compile), download.  The first call also
+    allocates the device and staging buffers; the second is reported."""
+    import ctypes as C
+
+    import numpy as np
+
+    from qclabpp_amd import _lib
+    from qclabpp_amd import gates as G
+
+    n = args.n
+    v = np.zeros(1 << n, np.complex128)
+    v[0] = 1
+    arr, cnt = G.records(circ)
+    secs = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        _lib.check(_lib.lib().qsv_host_simulate(v.ctypes.data_as(C.c_void_p), n, _lib.QSV_C128, arr, cnt,
+                                                 _lib.QSV_TILED_JIT))
+        secs.append(time.perf_counter() - t0)
+    _lib.check(_lib.lib().qsv_release_cache())
+    t = secs[-1]
+    xfer = max(t - ms_jit / 1e3, 1e-9)
+    return {"workload": f"brickwork n={n} depth={args.depth}, complex128, host vector in and out",
+            "s": t, "first_call_s": secs[0], "gate_passes_per_s_incl_pcie": len(circ) / t,
+            "transfer_s": xfer, "pcie_GBps_both_ways": 2 * v.nbytes / xfer / 1e9}
 
 
 def batch_leg(args):
@@ -487,6 +519,8 @@ def main():
                            "kernel": "k_fused_mfma<double,5>", "alg_bytes_per_launch": 2 * S}
     del psi
     torch.cuda.empty_cache()
+    if args.host_entry:
+        out["host_entry"] = host_entry_leg(args, circ, ms_jit)
     if args.adjoint_n > 0:
         out["adjoint"] = adjoint_leg(args)
         torch.cuda.empty_cache()
