@@ -203,6 +203,13 @@ typedef struct qsv_program_info {
 int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,
                        int diag_k, qsv_program** out);
 int qsv_program_run(const qsv_program* prog, void* psi, void* stream);
+/* Build check of QSV_TILED_JIT without a GPU: schedule the circuit into
+ * tiled passes, generate every pass kernel and compile it for gfx950 with
+ * hiprtc, without loading it.  *code_bytes (optional) receives the total
+ * code-object size and *npasses (optional) the pass count.  No counterpart
+ * in the reference; for build systems and CI hosts without a device.      */
+int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
+                                size_t* code_bytes, int64_t* npasses);
 int qsv_program_info_get(const qsv_program* prog, qsv_program_info* out);
 /* Launch i of one run (0-based, launch order): the qubits it acts on
  * (ascending, k of them), and its operator as interleaved complex doubles --

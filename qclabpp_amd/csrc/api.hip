@@ -217,6 +217,26 @@ int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, i
   return QSV_OK;
 }
 
+int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
+                                size_t* code_bytes, int64_t* npasses) {
+  if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
+  if (n < kQTileBits || n > 40) return fail(QSV_EINVAL, "tiled passes need n in [12, 40]");
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  std::vector<KOp> lowered(ngates);
+  for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, lowered[i]));
+  QProgram qp;
+  QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qp));
+  size_t total = 0;
+  for (size_t i = 0; i < qp.passes.size(); ++i) {
+    std::vector<char> code;
+    QSV_TRY(qtile_jit_compile(qtile_jit_source(qp, dtype == QSV_C128, i, i + 1), code));
+    total += code.size();
+  }
+  if (code_bytes) *code_bytes = total;
+  if (npasses) *npasses = int64_t(qp.passes.size());
+  return QSV_OK;
+}
+
 int qsv_program_run(const qsv_program* prog, void* psi, void* stream) {
   if (!prog) return fail(QSV_EINVAL, "null program");
   QSV_TRY(check_state_args(psi, prog->p.n, prog->p.dtype));

@@ -1,0 +1,47 @@
+"""The specialised tiled passes (QSV_TILED_JIT) generate and compile for
+gfx950 without a GPU: every pass of several circuits goes through source
+generation and hiprtc here, so a generator error fails the CPU suite."""
+from __future__ import annotations
+
+import ctypes as C
+
+import pytest
+
+from qclabpp_amd import _lib
+from qclabpp_amd import gates as G
+from qclabpp_amd import workloads as W
+from tests.test_gpu_tiled import every_kind
+
+
+def _check(n, circ, dtype):
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("engine library not built")
+    arr, cnt = G.records(circ)
+    size = C.c_size_t()
+    npass = C.c_int64()
+    st = _lib.lib().qsv_tiled_jit_compile_check(n, dtype, arr, cnt, C.byref(size), C.byref(npass))
+    if st == _lib.QSV_EUNSUP and b"hiprtc" in _lib.lib().qsv_last_error():
+        pytest.skip("hiprtc not loadable here")
+    _lib.check(st)
+    assert npass.value >= 1 and size.value > 0
+    return npass.value
+
+
+@pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
+def test_every_gate_kind_compiles(dtype):
+    # every gate class, both control states, Op N/C: exercises every op form
+    _check(13, every_kind(13, 13), dtype)
+
+
+def test_brickwork_and_dense_passes_compile():
+    n = 14
+    circ = W.brickwork(n, 6, seed=4) + [G.RotationXX(0, n - 1, 0.3), G.iSWAP(2, 9), G.CX(5, 6, 0)]
+    assert _check(n, circ, _lib.QSV_C128) >= 1
+
+
+def test_bad_arguments_are_rejected():
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("engine library not built")
+    lib = _lib.lib()
+    assert lib.qsv_tiled_jit_compile_check(8, _lib.QSV_C128, None, 0, None, None) == _lib.QSV_EINVAL
+    assert lib.qsv_tiled_jit_compile_check(14, 7, None, 0, None, None) == _lib.QSV_EDTYPE
