@@ -261,9 +261,9 @@ namespace {
 // One compile with a given rule for ties between tile candidates that
 // absorb equally many pending gates: strategy 0 takes the highest address
 // bit, 1 the lowest, others a pseudo-random one (seeded by the strategy).
-int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int strategy,
+int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, int strategy,
                        QProgram& out) {
-  constexpr int T = kQTileBits, RB = kQRegBits, NR = 1 << kQRegBits;
+  constexpr int RB = kQRegBits, NR = 1 << kQRegBits;
   uint64_t rng = 0x9E3779B97F4A7C15ull * uint64_t(strategy + 1);
   auto next = [&]() {
     rng ^= rng << 13;
@@ -541,9 +541,12 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int str
 
 }  // namespace
 
-int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out) {
-  if (n < kQTileBits)
-    return fail(QSV_EUNSUP, "tiled execution needs at least " + std::to_string(kQTileBits) + " qubits");
+int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out, int tile_bits) {
+  if (tile_bits < kQTileBits || tile_bits > kQTileBitsMax)
+    return fail(QSV_EINVAL, "tile bits must be in [" + std::to_string(kQTileBits) + ", " +
+                                std::to_string(kQTileBitsMax) + "]");
+  if (n < tile_bits)
+    return fail(QSV_EUNSUP, "tiled execution needs at least " + std::to_string(tile_bits) + " qubits");
   for (const KOp& op : in)
     if (op.kind == K_DENSEK || op.kind == K_DIAGK || op.nq > 2)
       return fail(QSV_EINVAL, "tiled execution takes unfused gates");
@@ -553,7 +556,8 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
   constexpr int kStrategies = 8;
   for (int st = 0; st < kStrategies; ++st) {
     QProgram cand;
-    QSV_TRY(compile_qtile_with(in, n, line_bits, st, cand));
+    QSV_TRY(compile_qtile_with(in, n, line_bits, tile_bits, st, cand));
+    cand.tile_bits = tile_bits;
     if (st == 0 || cand.passes.size() < out.passes.size() ||
         (cand.passes.size() == out.passes.size() && cand.phases.size() < out.phases.size()))
       out = std::move(cand);

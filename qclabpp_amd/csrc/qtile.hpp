@@ -21,6 +21,7 @@ namespace qsv {
 constexpr int kQTileBits = 12;     // tile: 2^12 amplitudes (64 KiB of complex<double>)
 constexpr int kQRegBits = 4;       // registers: 16 amplitudes per thread
 constexpr int kQThreads = 256;     // (1 << kQTileBits) / (1 << kQRegBits)
+constexpr int kQTileBitsMax = 13;  // compiled passes may use 2^13-amplitude tiles
 
 // Ops act on a thread's 16 registers.  Inside a phase the host tracks which
 // register holds which logical amplitude (an affine relabeling over GF(2)):
@@ -81,11 +82,15 @@ struct QProgram {
   std::vector<QPhase> phases;
   std::vector<QOp> ops;
   size_t gates_absorbed = 0;
+  int tile_bits = kQTileBits;
 };
 
-// Compile lowered, unfused gates (n > kQTileBits) into passes and phases.
+// Compile lowered, unfused gates (n >= tile_bits) into passes and phases.
 // line_bits: address bits inside one 128-byte line, always kept in the tile.
-int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out);
+// tile_bits: address bits per tile (kQTileBits for the interpreting kernel,
+// up to kQTileBitsMax for compiled passes).
+int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out,
+                  int tile_bits = kQTileBits);
 
 struct QTileLaunch {
   void* psi;

@@ -152,9 +152,9 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128) {
 std::string u64(uint64_t x) { return std::to_string(x) + "ull"; }
 
 // Global offset (in amplitudes) of tile position p of pass P.
-uint64_t gofs(const QPass& P, uint32_t p) {
+uint64_t gofs(const QPass& P, int T, uint32_t p) {
   uint64_t o = 0;
-  for (int b = 0; b < kQTileBits; ++b)
+  for (int b = 0; b < T; ++b)
     if (p >> b & 1u) o |= uint64_t(1) << P.tbits[b];
   return o;
 }
@@ -162,10 +162,10 @@ uint64_t gofs(const QPass& P, uint32_t p) {
 // Expression for the global offset of this thread's share of a phase:
 // thread index bit j sits on the j-th lowest tile bit outside the phase's
 // registers.
-std::string thread_gofs(const QPass& P, uint32_t regmask) {
+std::string thread_gofs(const QPass& P, int T, uint32_t regmask) {
   std::ostringstream e;
   e << "(0ull";
-  for (int b = 0, j = 0; b < kQTileBits && j < 8; ++b)
+  for (int b = 0, j = 0; b < T && j < T - kQRegBits; ++b)
     if (!(regmask >> b & 1u)) {
       e << " | ((unsigned long long)((tid >> " << j << ") & 1u) << " << int(P.tbits[b]) << ")";
       ++j;
@@ -183,18 +183,23 @@ std::string thread_gofs(const QPass& P, uint32_t regmask) {
 // both ends direct never touches LDS.
 void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) {
   const QPass& P = qp.passes[pi];
+  const int T = qp.tile_bits;
+  const unsigned threads = 1u << (T - kQRegBits);
+  const uint32_t tile_order = ((1u << kQRegBits) - 1) << (T - kQRegBits);  // registers j of position tid + threads*j
   const int K = P.phase_end - P.phase_begin;
   const QPhase& F = qp.phases[P.phase_begin];
   const QPhase& L = qp.phases[P.phase_end - 1];
   const uint32_t line = c128 ? 7u : 15u;  // tile bits of one 128-byte line
   const bool din = !(F.regmask & line);
   const bool dout = !(L.regmask & line);
-  // waves per SIMD the register budget is sized for: a complex128 tile
-  // (64 KiB of LDS) allows two blocks per CU, a complex64 one three
-  const int wpe = c128 ? 2 : 3;
-  s << "extern \"C\" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(" << wpe << ")))\n"
+  // waves per SIMD the register budget is sized for, from the blocks per CU
+  // that LDS allows (160 KiB): complex128 tiles of 2^12 (64 KiB, 2 blocks of
+  // 4 waves), complex64 tiles of 2^12 (32 KiB, 3 blocks of 4 waves) or 2^13
+  // (64 KiB, 2 blocks of 8 waves)
+  const int wpe = c128 ? (T == 12 ? 2 : 1) : (T == 12 ? 3 : 4);
+  s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(" << wpe << ")))\n"
     << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
-  if (!(din && dout && K == 1)) s << "  __shared__ C tile[" << (1 << kQTileBits) << "];\n";
+  if (!(din && dout && K == 1)) s << "  __shared__ C tile[" << (1u << T) << "];\n";
   s << "  const unsigned tid = threadIdx.x;\n"
     << "  auto tile_base = [&](unsigned long long t) {\n"
     << "    unsigned long long b = 0;\n";
@@ -206,14 +211,14 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
   s << "    return b;\n"
     << "  };\n";
   // offsets of the 16 amplitudes a thread loads (and stores) at the ends:
-  // the first (last) phase's registers, or tile positions tid + 256*j
+  // the first (last) phase's registers, or tile positions tid + threads*j
   std::vector<uint64_t> in_ofs(NR), out_ofs(NR);
   for (int i = 0; i < NR; ++i) {
-    in_ofs[i] = din ? gofs(P, qtile_swizzle(F.regpos[i])) : gofs(P, uint32_t(kQThreads * i));
-    out_ofs[i] = dout ? gofs(P, qtile_swizzle(L.regpos_out[i])) : gofs(P, uint32_t(kQThreads * i));
+    in_ofs[i] = din ? gofs(P, T, qtile_swizzle(F.regpos[i])) : gofs(P, T, threads * uint32_t(i));
+    out_ofs[i] = dout ? gofs(P, T, qtile_swizzle(L.regpos_out[i])) : gofs(P, T, threads * uint32_t(i));
   }
-  s << "  const unsigned long long gin = " << (din ? thread_gofs(P, F.regmask) : thread_gofs(P, 0xF00u)) << ";\n"
-    << "  const unsigned long long gout = " << (dout ? thread_gofs(P, L.regmask) : thread_gofs(P, 0xF00u)) << ";\n";
+  s << "  const unsigned long long gin = " << thread_gofs(P, T, din ? F.regmask : tile_order) << ";\n"
+    << "  const unsigned long long gout = " << thread_gofs(P, T, dout ? L.regmask : tile_order) << ";\n";
   auto arr = [&](const char* name, const std::vector<uint64_t>& v) {
     s << "  const unsigned long long " << name << "[16] = {";
     for (int i = 0; i < NR; ++i) s << (i ? ", " : "") << u64(v[i]);
@@ -241,7 +246,7 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
       << "    for (int j = 0; j < 16; ++j) v[j] = pre[j];\n";
   } else {
     s << "#pragma unroll\n"
-      << "    for (int j = 0; j < 16; ++j) tile[tslot ^ swz(256u * j)] = pre[j];\n"
+      << "    for (int j = 0; j < 16; ++j) tile[tslot ^ swz(" << threads << "u * j)] = pre[j];\n"
       << "    __syncthreads();\n";
   }
   s << "    {\n"
@@ -269,7 +274,7 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
         << "      asm volatile(\"\" : \"+v\"(x));\n";
       // spread the thread index over the tile bits outside the registers:
       // insert a zero at each register bit, lowest first
-      for (int b = 0; b < kQTileBits; ++b)
+      for (int b = 0; b < T; ++b)
         if (Q.regmask >> b & 1u)
           s << "      x = ((x >> " << b << ") << " << (b + 1) << ") | (x & " << ((1u << b) - 1) << "u);\n";
       s << "      const unsigned tp = swz(x);\n";
@@ -294,7 +299,7 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
     s << "    {\n"
       << "      const unsigned long long bo = base + gout;\n"
       << "#pragma unroll\n"
-      << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], tile[tslot ^ swz(256u * j)]);\n"
+      << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], tile[tslot ^ swz(" << threads << "u * j)]);\n"
       << "      __syncthreads();\n"
       << "    }\n";
   }
@@ -458,6 +463,7 @@ int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
   out.mods = mods;
   out.fn.resize(np);
   out.ntiles.resize(np);
+  out.threads = 1u << (qp.tile_bits - kQRegBits);
   for (size_t g = 0; g < groups; ++g)
     for (size_t i = cut[g]; i < cut[g + 1]; ++i) {
       const std::string name = "qtp_" + std::to_string(i);
@@ -474,7 +480,7 @@ int qtile_jit_launch(const QJit& j, size_t i, void* psi, const QOp* ops, hipStre
   const unsigned grid = unsigned(ntiles < want ? ntiles : want);
   const void* cf = ops;
   void* args[] = {&psi, &cf, &ntiles};
-  QSV_CHECK_HIP(hipModuleLaunchKernel(j.fn[i], grid, 1, 1, kQThreads, 1, 1, 0, st, args, nullptr));
+  QSV_CHECK_HIP(hipModuleLaunchKernel(j.fn[i], grid, 1, 1, j.threads, 1, 1, 0, st, args, nullptr));
   return QSV_OK;
 }
 

@@ -24,7 +24,14 @@ struct QJit {
   std::vector<std::shared_ptr<QJitModule>> mods;
   std::vector<hipFunction_t> fn;  // per pass
   std::vector<uint64_t> ntiles;   // per pass
+  unsigned threads = kQThreads;   // block size: 2^(tile bits - register bits)
 };
+
+// Tile bits of compiled passes: 2^13-amplitude tiles for complex64 (64 KiB
+// of LDS, as a complex128 tile of 2^12).
+inline int jit_tile_bits(int dtype, int n) {
+  return dtype == QSV_C64 && n >= kQTileBitsMax ? kQTileBitsMax : kQTileBits;
+}
 
 // Source of the kernels for passes [p0, p1) of qp (test hook and build step).
 std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1);

@@ -105,3 +105,15 @@ def test_jit_structure_shared_across_angles():
     assert np.abs(run(pa, psi) - oracle.apply(n, a, psi)).max() < 1e-12
     assert np.abs(run(pb, psi) - oracle.apply(n, b, psi)).max() < 1e-12
     assert t2 - t1 < 0.5 * (t1 - t0)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_tiled_brickwork_c64(form):
+    """complex64 passes (compiled ones use 2^13-amplitude tiles) against the
+    oracle, through the same circuit in complex128 as well."""
+    n = 20
+    circ = W.brickwork(n, 20, seed=8) + W.tfim_ansatz(n, 1)
+    psi = random_state(n, 3, np.complex128)
+    want = oracle.apply(n, circ, psi)
+    got = run(E.Program(n, circ, fuse_k=form, dtype="c64"), psi.astype(np.complex64))
+    assert np.abs(got - want).max() < 5e-5
