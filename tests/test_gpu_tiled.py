@@ -117,3 +117,31 @@ def test_tiled_brickwork_c64(form):
     want = oracle.apply(n, circ, psi)
     got = run(E.Program(n, circ, fuse_k=form, dtype="c64"), psi.astype(np.complex64))
     assert np.abs(got - want).max() < 5e-5
+
+
+def _structured(n):
+    """Circuits whose passes take each load/store route of the compiled
+    kernels: gates only on the line qubits (first and last phases gather
+    and scatter through LDS), one gate far from them (a one-phase pass with
+    no LDS at all), and mixes whose first or last phase holds line bits."""
+    hi = [0, 1, 2, 3]
+    line = [n - 1, n - 2, n - 3]
+    rot = lambda q, t: [G.RotationX(q, t), G.RotationY(q, 0.7 * t), G.RotationZ(q, 0.3 * t)]
+    c_line = sum((rot(q, 0.4 + q) for q in line), []) + [G.CX(n - 1, n - 2), G.CZ(n - 3, n - 1)]
+    c_one = [G.RotationY(0, 0.9)]
+    c_hi_then_line = sum((rot(q, 0.2 * q + 0.1) for q in hi), []) + [G.CX(0, 3)] + c_line
+    c_line_then_hi = c_line + sum((rot(q, 0.5 * q + 0.3) for q in hi), []) + [G.SWAP(1, 2)]
+    return {"line_only": c_line, "one_gate": c_one, "high_then_line": c_hi_then_line,
+            "line_then_high": c_line_then_hi}
+
+
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+@pytest.mark.parametrize("name", ["line_only", "one_gate", "high_then_line", "line_then_high"])
+def test_jit_load_store_routes(name, dt):
+    n = 15
+    circ = _structured(n)[name]
+    psi = random_state(n, 12, np.complex128)
+    want = oracle.apply(n, circ, psi)
+    x = psi if dt == "c128" else psi.astype(np.complex64)
+    got = run(E.Program(n, circ, fuse_k=E.TILED_JIT, dtype=dt), x)
+    assert np.abs(got - want).max() < (1e-12 if dt == "c128" else 5e-5)
