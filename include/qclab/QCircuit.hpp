@@ -63,11 +63,31 @@ class QCircuit : public QObject<T>, public QAdjustable {
   }
   void setQubits(const int*) override { assert(false); }
 
-  /// Unitary of the circuit (host arithmetic, MSB kron convention).
+  /// Unitary of the circuit (MSB kron convention): on the GPU from
+  /// backend::matrix_gpu_min() qubits up, host arithmetic below.
   dense::SquareMatrix<T> matrix() const override {
+    return nbQubits_ >= backend::matrix_gpu_min() ? matrix_gpu() : matrix_host();
+  }
+  /// The reference's construction: every gate's apply(Side::Right, ...) on
+  /// an identity, in host arithmetic (QCircuit.hpp:121-127).
+  dense::SquareMatrix<T> matrix_host() const {
     auto mat = dense::eye<T>(int64_t(1) << nbQubits_);
     for (auto it = begin(); it != end(); ++it) (*it)->apply(Side::Right, Op::NoTrans, nbQubits_, mat);
     return mat;
+  }
+  /// The same unitary built on the GPU: the identity as a 2k-qubit state,
+  /// the gates applied to its row qubits (one engine call, one download).
+  dense::SquareMatrix<T> matrix_gpu() const {
+    dense::SquareMatrix<T> mat(int64_t(1) << nbQubits_);
+    backend::circuit_matrix(mat.ptr(), nbQubits_, gpu_records(), backend::fuse_k());
+    return mat;
+  }
+  /// Engine records of the gates as matrix() applies them: in order, with
+  /// offset 0 (nested circuits add their own offsets).
+  std::vector<qsv_gate> gpu_records() const {
+    std::vector<qsv_gate> g;
+    for (auto it = begin(); it != end(); ++it) (*it)->records(g, Op::NoTrans, 0);
+    return g;
   }
 
   /// Applies the circuit (with `op`) to `vector` on the GPU.
@@ -177,9 +197,10 @@ class QCircuit : public QObject<T>, public QAdjustable {
 };
 
 /// Frobenius norm of the difference of two circuits' unitaries, computed
-/// with a max-abs scale against overflow (reference QCircuit.hpp:406-430).
+/// with a max-abs scale against overflow (reference QCircuit.hpp:406-430),
+/// in host arithmetic.
 template <typename C1, typename C2>
-double nrmF(const C1& circuit1, const C2& circuit2) {
+double nrmF_host(const C1& circuit1, const C2& circuit2) {
   assert(circuit1.nbQubits() == circuit2.nbQubits());
   const auto m1 = circuit1.matrix();
   const auto m2 = circuit2.matrix();
@@ -194,6 +215,21 @@ double nrmF(const C1& circuit1, const C2& circuit2) {
       sum += re * re + im * im;
     }
   return scale * std::sqrt(sum);
+}
+
+template <typename C1, typename C2>
+double nrmF(const C1& circuit1, const C2& circuit2) {
+  return nrmF_host(circuit1, circuit2);
+}
+
+/// nrmF of two circuits: from backend::matrix_gpu_min() qubits up both
+/// unitaries are built and reduced on the GPU, with the same scaling.
+template <typename T, typename G1, typename G2>
+double nrmF(const QCircuit<T, G1>& circuit1, const QCircuit<T, G2>& circuit2) {
+  assert(circuit1.nbQubits() == circuit2.nbQubits());
+  if (circuit1.nbQubits() < backend::matrix_gpu_min()) return nrmF_host(circuit1, circuit2);
+  return backend::circuit_nrmf<T>(circuit1.nbQubits(), circuit1.gpu_records(), circuit2.gpu_records(),
+                                  backend::fuse_k());
 }
 
 }  // namespace qclab

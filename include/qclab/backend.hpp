@@ -92,5 +92,27 @@ void run_device(T* v, int n, const std::vector<qsv_gate>& g, int fuse) {
   check(qsv_dev_simulate(v, n, dtype<T>(), g.data(), g.size(), fuse, nullptr));
 }
 
+/// QCircuit::matrix() and nrmF of two circuits run on the GPU from this many
+/// qubits up (environment QCLAB_MATRIX_GPU_MIN, default 6); smaller unitaries
+/// use host arithmetic, as the reference does for all sizes.
+inline int matrix_gpu_min() {
+  const char* e = std::getenv("QCLAB_MATRIX_GPU_MIN");
+  return e ? std::atoi(e) : 6;
+}
+
+/// The 2^k x 2^k unitary of k-qubit records into column-major host memory.
+template <typename T>
+void circuit_matrix(T* out, int k, const std::vector<qsv_gate>& g, int fuse) {
+  check(qsv_host_circuit_matrix(out, k, dtype<T>(), g.data(), g.size(), fuse));
+}
+
+/// nrmF of two k-qubit record lists, both unitaries built and reduced on the GPU.
+template <typename T>
+double circuit_nrmf(int k, const std::vector<qsv_gate>& g1, const std::vector<qsv_gate>& g2, int fuse) {
+  double out = 0.0;
+  check(qsv_circuit_nrmf(k, dtype<T>(), g1.data(), g1.size(), g2.data(), g2.size(), fuse, &out));
+  return out;
+}
+
 }  // namespace backend
 }  // namespace qclab

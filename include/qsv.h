@@ -325,6 +325,37 @@ int qsv_dev_simulate(void* psi, int n, int dtype, const qsv_gate* gates, size_t 
                      int fuse_k, void* stream);
 int qsv_release_cache(void);
 
+/* ---- unitaries (QCircuit::matrix, nrmF) -----------------------------------
+ * Replace QCircuit<T>::matrix() (include/qclab/QCircuit.hpp:121-127, the
+ * gates' apply(Side::Right, ...) on an identity, e.g.
+ * src/qgates/QGate1.cpp:27-82) and nrmF (QCircuit.hpp:406-430).
+ * `mat` receives the 2^k x 2^k unitary of the k-qubit gate list, column-
+ * major (4^k amplitudes), in the MSB kron convention of matrix().  The
+ * matrix is simulated as a 2k-qubit state (entry (r, c) at c*2^k + r: the
+ * row index is qubits k..2k-1), so every compiled form applies (fuse_k as
+ * for qsv_host_simulate).  Gate qubits must lie in [0, k), k in [1, 20].
+ * QGate2 records (QSV_GATE_U2) act in the kron order of QGate2::matrix(),
+ * as in the reference's matrix() (src/qgates/QGate2.cpp:32-109), not in
+ * the transposed vector basis of its simulate() (src/qgates/apply.hpp:
+ * 289-308); every other gate acts identically in both.                    */
+int qsv_dev_circuit_matrix(void* mat, int k, int dtype, const qsv_gate* gates, size_t ngates,
+                           int fuse_k, void* stream);
+/* Same into host memory (through the cached device buffer, as
+ * qsv_host_simulate).                                                     */
+int qsv_host_circuit_matrix(void* mat, int k, int dtype, const qsv_gate* gates, size_t ngates,
+                            int fuse_k);
+/* Frobenius norm of a - b over `count` amplitudes with nrmF's max-abs
+ * scaling: scale = max |a_i - b_i| (1 when all are equal), result
+ * scale * sqrt(sum |a_i - b_i|^2 / scale^2), accumulated in FP64 in a fixed
+ * order (QCircuit.hpp:406-430).                                            */
+int qsv_dev_nrmf(const void* a, const void* b, uint64_t count, int dtype, double* out,
+                 void* stream);
+/* nrmF(circuit1, circuit2) for two k-qubit gate lists: both unitaries are
+ * built on the GPU (two 4^k-amplitude device buffers, freed on return) and
+ * reduced there; only the FP64 result comes back.                         */
+int qsv_circuit_nrmf(int k, int dtype, const qsv_gate* gates1, size_t ngates1,
+                     const qsv_gate* gates2, size_t ngates2, int fuse_k, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,13 +8,14 @@ and only to check or time against it.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 from qclabpp_amd._lib import QsvGate
-from qclabpp_amd.gates import records
+from qclabpp_amd.gates import U2, records
 
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
@@ -116,6 +117,30 @@ def apply(n: int, gates, psi: np.ndarray, op: str | None = None) -> np.ndarray:
         st = oracle_lib().oracle_apply_gates_op(n, _dtype_code(out), arr, cnt, ord(op), _ptr(out))
     if st:
         raise RuntimeError(f"oracle status {st}")
+    return out
+
+
+def matrix(k: int, gates, dtype=np.complex128) -> np.ndarray:
+    """QCircuit::matrix() restated (include/qclab/QCircuit.hpp:121-127): the
+    2^k x 2^k unitary, column c = the gates applied to |c> by the restated
+    sweeps.  A QGate2 record acts there in the transposed vector basis of
+    src/qgates/apply.hpp:289-308, while matrix() applies QGate2::matrix() in
+    kron order (src/qgates/QGate2.cpp:32-109); so each QGate2 record is
+    given P M P (basis states 01 and 10 exchanged), which the vector sweep
+    turns back into M in kron order.  Every other gate acts identically in
+    both.  Pure-Python column loop: for k <= 8."""
+    N = 1 << k
+    recs = []
+    for g in gates:
+        if g.kind == U2:
+            p = [0, 2, 1, 3]
+            g = dataclasses.replace(g, mat=np.asarray(g.mat, dtype=np.complex128)[np.ix_(p, p)])
+        recs.append(g)
+    out = np.zeros((N, N), dtype=dtype)
+    for c in range(N):
+        e = np.zeros(N, dtype=dtype)
+        e[c] = 1
+        out[:, c] = apply(k, recs, e)
     return out
 
 

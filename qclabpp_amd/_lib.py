@@ -161,6 +161,10 @@ SIGNATURES = {
                                 C.POINTER(QsvBatchOpts), C.POINTER(QsvTuneResult), _I,
                                 C.POINTER(_I), _P]),
     "qsv_dev_max_abs_diff": (_I, [_P, _P, _U64, _I, _DP, _P]),
+    "qsv_dev_circuit_matrix": (_I, [_P, _I, _I, _GP, _SZ, _I, _P]),
+    "qsv_host_circuit_matrix": (_I, [_P, _I, _I, _GP, _SZ, _I]),
+    "qsv_dev_nrmf": (_I, [_P, _P, _U64, _I, _DP, _P]),
+    "qsv_circuit_nrmf": (_I, [_I, _I, _GP, _SZ, _GP, _SZ, _I, _DP]),
     "qsv_dev_adjoint_gradient": (_I, [_P, _P, _I, _I, _GP, _SZ, _DP, _SZ, C.POINTER(QsvPauliTerm),
                                       _SZ, _I, _DP, _DP, _P]),
 }

@@ -10,6 +10,7 @@
 #include "plain.hpp"
 #include "fusion.hpp"
 #include "program.hpp"
+#include "matrix.hpp"
 
 struct qsv_program {
   qsv::Program p;
@@ -345,6 +346,67 @@ int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_
   QSV_CHECK_HIP(hipStreamSynchronize(nullptr));
   QSV_TRY(download(host, g_cache.ptr, bytes));
   return QSV_OK;
+}
+
+int qsv_dev_circuit_matrix(void* mat, int k, int dtype, const qsv_gate* gates, size_t ngates,
+                           int fuse_k, void* stream) {
+  if (!mat) return fail(QSV_EINVAL, "null matrix pointer");
+  QSV_TRY(check_state_args(mat, 2 * k, dtype));
+  std::vector<qsv_gate> recs;
+  QSV_TRY(matrix_records(k, gates, ngates, recs));
+  auto st = static_cast<hipStream_t>(stream);
+  QSV_TRY(launch_set_identity(mat, k, dtype, st));
+  if (recs.empty()) {
+    QSV_CHECK_HIP(hipStreamSynchronize(st));
+    return QSV_OK;
+  }
+  return qsv_dev_simulate(mat, 2 * k, dtype, recs.data(), recs.size(), fuse_k, stream);
+}
+
+int qsv_host_circuit_matrix(void* mat, int k, int dtype, const qsv_gate* gates, size_t ngates,
+                            int fuse_k) {
+  if (!mat) return fail(QSV_EINVAL, "null matrix pointer");
+  QSV_TRY(check_state_args(mat, 2 * k, dtype));
+  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
+      fuse_k != QSV_TILED_JIT)
+    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
+  std::vector<qsv_gate> recs;
+  QSV_TRY(matrix_records(k, gates, ngates, recs));
+  Program prog;
+  QSV_TRY(prog.build(2 * k, dtype, recs.data(), recs.size(), fuse_k, 0));
+  const size_t bytes = (size_t(1) << (2 * k)) * elem_bytes(dtype);
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  QSV_TRY(reserve_cache(bytes));
+  QSV_TRY(launch_set_identity(g_cache.ptr, k, dtype, nullptr));
+  if (!recs.empty()) QSV_TRY(prog.run(g_cache.ptr, nullptr));
+  QSV_CHECK_HIP(hipStreamSynchronize(nullptr));
+  QSV_TRY(download(mat, g_cache.ptr, bytes));
+  return QSV_OK;
+}
+
+int qsv_dev_nrmf(const void* a, const void* b, uint64_t count, int dtype, double* out, void* stream) {
+  return dev_nrmf(a, b, count, dtype, out, static_cast<hipStream_t>(stream));
+}
+
+int qsv_circuit_nrmf(int k, int dtype, const qsv_gate* gates1, size_t ngates1, const qsv_gate* gates2,
+                     size_t ngates2, int fuse_k, double* out) {
+  if (!out) return fail(QSV_EINVAL, "null result pointer");
+  if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
+  if (k < 1 || 2 * k > 40) return fail(QSV_EINVAL, "matrix qubits must be in [1, 20]");
+  const size_t bytes = (size_t(1) << (2 * k)) * elem_bytes(dtype);
+  void* m[2] = {nullptr, nullptr};
+  for (int i = 0; i < 2; ++i)
+    if (hipMalloc(&m[i], bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      if (m[0]) (void)hipFree(m[0]);
+      return fail(QSV_ENOMEM, "nrmF: device allocation of two " + std::to_string(bytes) + "-byte matrices failed");
+    }
+  int rc = qsv_dev_circuit_matrix(m[0], k, dtype, gates1, ngates1, fuse_k, nullptr);
+  if (rc == QSV_OK) rc = qsv_dev_circuit_matrix(m[1], k, dtype, gates2, ngates2, fuse_k, nullptr);
+  if (rc == QSV_OK) rc = dev_nrmf(m[0], m[1], uint64_t(1) << (2 * k), dtype, out, nullptr);
+  (void)hipFree(m[0]);
+  (void)hipFree(m[1]);
+  return rc;
 }
 
 int qsv_release_cache(void) {

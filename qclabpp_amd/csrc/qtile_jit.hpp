@@ -10,6 +10,7 @@
 // state once per gate); the result is the product of the same lowered gates.
 #pragma once
 
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -28,9 +29,16 @@ struct QJit {
 };
 
 // Tile bits of compiled passes: 2^13-amplitude tiles for complex64 (64 KiB
-// of LDS, as a complex128 tile of 2^12).
+// of LDS, as a complex128 tile of 2^12).  QSV_JIT_TILE_BITS=12 or 13
+// overrides the choice (a complex128 tile of 2^13 takes 128 KiB of LDS, one
+// block per CU).
 inline int jit_tile_bits(int dtype, int n) {
-  return dtype == QSV_C64 && n >= kQTileBitsMax ? kQTileBitsMax : kQTileBits;
+  int t = dtype == QSV_C64 && n >= kQTileBitsMax ? kQTileBitsMax : kQTileBits;
+  if (const char* e = std::getenv("QSV_JIT_TILE_BITS")) {
+    const int v = std::atoi(e);
+    if (v >= kQTileBits && v <= kQTileBitsMax && n >= v) t = v;
+  }
+  return t;
 }
 
 // Source of the kernels for passes [p0, p1) of qp (test hook and build step).

@@ -102,6 +102,52 @@ AUTO = _lib.QSV_AUTO
 TILED_JIT = _lib.QSV_TILED_JIT
 
 
+def circuit_matrix(k: int, gates, dtype=None, fuse_k: int = AUTO, stream=None):
+    """QCircuit::matrix() on the GPU (reference include/qclab/QCircuit.hpp:
+    121-127): the 2^k x 2^k unitary of a k-qubit gate list as a device tensor
+    M with M[r, c] = <r|U|c> (MSB kron convention).  The engine fills a
+    column-major buffer; the returned tensor is its transposed view."""
+    torch = _torch()
+    dtype = dtype or torch.complex128
+    dt = dtype_code(torch.empty(0, dtype=dtype))
+    N = 1 << k
+    buf = torch.empty(N * N, dtype=dtype, device="cuda")
+    arr, cnt = records(gates)
+    check(lib().qsv_dev_circuit_matrix(C.c_void_p(buf.data_ptr()), k, dt, arr, cnt, fuse_k,
+                                       stream_handle(stream)))
+    return buf.view(N, N).t()
+
+
+def nrmf(a, b, stream=None) -> float:
+    """nrmF's scaled Frobenius norm of a - b (reference QCircuit.hpp:406-430)
+    for two device tensors of equal size, dtype and layout."""
+    if a.shape != b.shape or a.dtype != b.dtype:
+        raise ValueError("nrmf: tensors differ in shape or dtype")
+    if not (a.is_cuda and b.is_cuda):
+        raise ValueError("nrmf: tensors must live on the GPU")
+    # the norm pairs elements by storage offset: both must be dense in the
+    # same layout (contiguous, or both transposed views of contiguous buffers)
+    dense = a.numel() == 0 or (a.stride() == b.stride() and
+                               (a.is_contiguous() or (a.dim() == 2 and a.t().is_contiguous())))
+    if not dense:
+        raise ValueError("nrmf: tensors must be dense with equal strides")
+    dt = dtype_code(a)
+    out = C.c_double()
+    check(lib().qsv_dev_nrmf(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), a.numel(), dt,
+                             C.byref(out), stream_handle(stream)))
+    return out.value
+
+
+def circuit_nrmf(k: int, gates1, gates2, dtype: str = "c128", fuse_k: int = AUTO) -> float:
+    """nrmF(circuit1, circuit2) with both unitaries built and reduced on the GPU."""
+    dt = QSV_C128 if dtype in ("c128", "complex128") else QSV_C64
+    a1, c1 = records(gates1)
+    a2, c2 = records(gates2)
+    out = C.c_double()
+    check(lib().qsv_circuit_nrmf(k, dt, a1, c1, a2, c2, fuse_k, C.byref(out)))
+    return out.value
+
+
 class Program:
     """A compiled circuit (qsv_program): plain (fuse_k=0), fused (1..5) or
     tiled passes (fuse_k=TILED)."""

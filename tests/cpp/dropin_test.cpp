@@ -146,6 +146,40 @@ class MyGate2 : public QGate2<T> {
   int q_[2];
 };
 
+// a user 2-qubit gate whose matrix changes under exchanging its qubits:
+// matrix() applies it in kron order, simulate() in QGate2's vector basis
+template <typename T>
+class MyGate2A : public QGate2<T> {
+ public:
+  MyGate2A(int a, int b) : q_{std::min(a, b), std::max(a, b)} {}
+  bool fixed() const override { return true; }
+  bool controlled() const override { return false; }
+  int qubit() const override { return q_[0]; }
+  std::vector<int> qubits() const override { return {q_[0], q_[1]}; }
+  void setQubits(const int* q) override { q_[0] = q[0]; q_[1] = q[1]; }
+  dense::SquareMatrix<T> matrix() const override {
+    dense::SquareMatrix<T> m(4, T(0));
+    m(0, 0) = T(1);
+    m(1, 3) = T(1);
+    m(2, 1) = T(0, 1);
+    m(3, 2) = T(0.6, 0.8);
+    return m;
+  }
+  int toQASM(std::ostream&, const int = 0) const override { return -1; }
+
+ protected:
+  bool equals(const QObject<T>& o) const override { return dynamic_cast<const MyGate2A<T>*>(&o); }
+  int q_[2];
+};
+
+template <typename T>
+double mat_diff(const dense::SquareMatrix<T>& a, const dense::SquareMatrix<T>& b) {
+  double m = 0;
+  for (int64_t c = 0; c < a.cols(); ++c)
+    for (int64_t r = 0; r < a.rows(); ++r) m = std::max(m, double(std::abs(a(r, c) - b(r, c))));
+  return m;
+}
+
 template <typename T>
 void gate_checks(const char* tn) {
   const int n = 6;
@@ -238,6 +272,48 @@ void circuit_checks(const char* tn) {
     for (int r = 0; r < 16; ++r) worst = std::max(worst, double(std::abs(e[r] - M(r, j))));
   }
   CHECK(worst < 10 * tol<T>(), "%s simulate(e_j) vs matrix(): %.3e", tn, worst);
+
+  // matrix() on the GPU (the identity as a 2k-qubit state) against the
+  // host construction, in every compiled form
+  {
+    QCircuit<T> m7 = sample_circuit<T>(n, 13);
+    m7.push_back(std::make_unique<MyGate1<T>>(3));
+    m7.push_back(std::make_unique<MyGate2A<T>>(2, 3));
+    m7.push_back(std::make_unique<MyGate2A<T>>(4, 6));
+    m7.push_back(std::make_unique<CRotationY<T>>(6, 0, real_t<T>(0.7), 0));
+    const auto Mh = m7.matrix_host();
+    for (const char* form : {"0", "5", "tiled", "auto"}) {
+      setenv("QCLAB_FUSE_K", form, 1);
+      const double d = mat_diff(m7.matrix_gpu(), Mh);
+      CHECK(d < 10 * tol<T>(), "%s matrix_gpu vs matrix_host (form %s): %.3e", tn, form, d);
+    }
+    unsetenv("QCLAB_FUSE_K");
+    // nested circuit with an offset and pointer gates; matrix() ignores the
+    // top circuit's own offset
+    const double dn = mat_diff(outer.matrix_gpu(), outer.matrix_host());
+    CHECK(dn < 10 * tol<T>(), "%s nested matrix_gpu vs matrix_host: %.3e", tn, dn);
+    QCircuit<T> shifted(4, 3);
+    shifted.push_back(std::make_unique<Hadamard<T>>(0));
+    shifted.push_back(std::make_unique<CX<T>>(0, 3));
+    shifted.push_back(std::make_unique<MyGate2A<T>>(1, 2));
+    const double ds = mat_diff(shifted.matrix_gpu(), shifted.matrix_host());
+    CHECK(ds < 10 * tol<T>(), "%s offset circuit matrix_gpu vs matrix_host: %.3e", tn, ds);
+    // nrmF: both unitaries and the reduction on the GPU, against the host
+    QCircuit<T> m7b = sample_circuit<T>(n, 13);
+    m7b.push_back(std::make_unique<MyGate1<T>>(3));
+    m7b.push_back(std::make_unique<MyGate2A<T>>(2, 3));
+    m7b.push_back(std::make_unique<MyGate2A<T>>(4, 6));
+    m7b.push_back(std::make_unique<CRotationY<T>>(6, 0, real_t<T>(0.75), 0));
+    const double fg = nrmF(m7, m7b), fz = nrmF(m7, m7);
+    setenv("QCLAB_MATRIX_GPU_MIN", "99", 1);
+    const double fh = nrmF(m7, m7b);
+    CHECK(m7 == m7 && m7 != m7b, "%s circuit equality by host matrix()", tn);
+    unsetenv("QCLAB_MATRIX_GPU_MIN");
+    CHECK(m7 == m7 && m7 != m7b, "%s circuit equality by GPU matrix()", tn);
+    const double rel = std::is_same_v<T, std::complex<double>> ? 1e-10 : 1e-3;
+    CHECK(fz == 0.0 && fg > 0 && std::abs(fg - fh) < rel * fh,
+          "%s nrmF GPU %.17g host %.17g self %.3g", tn, fg, fh, fz);
+  }
 
   // device-resident vector
   auto host = v0;

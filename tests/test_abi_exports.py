@@ -49,3 +49,15 @@ def test_calls_that_need_no_gpu():
     assert b"null" in lib.qsv_last_error()
     buf = (ctypes.c_double * 16)()
     assert lib.qsv_host_simulate(buf, 3, 9, None, 0, 0) == 5  # QSV_EDTYPE
+
+
+def test_matrix_argument_checks_need_no_gpu():
+    from qclabpp_amd import gates as G
+    lib = _loaded()
+    buf = (ctypes.c_double * 8)()
+    arr, cnt = G.records([G.Hadamard(2)])
+    assert lib.qsv_host_circuit_matrix(buf, 2, _lib.QSV_C128, arr, cnt, 0) == _lib.QSV_EQUBIT
+    assert lib.qsv_host_circuit_matrix(buf, 21, _lib.QSV_C128, arr, cnt, 0) == _lib.QSV_EINVAL
+    assert lib.qsv_host_circuit_matrix(None, 2, _lib.QSV_C128, arr, cnt, 0) == _lib.QSV_EINVAL
+    out = ctypes.c_double()
+    assert lib.qsv_circuit_nrmf(2, 9, arr, cnt, arr, cnt, 0, ctypes.byref(out)) == _lib.QSV_EDTYPE
