@@ -198,6 +198,8 @@ typedef struct qsv_program_info {
   double  alg_bytes;   /* algorithmic HBM bytes of one run              */
   double  flops;       /* useful flops of the dense fused blocks        */
   int64_t ntiled_phases; /* tiled programs: register phases of one run  */
+  int64_t ntiled_relabel; /* compiled tiled passes that write their tile
+                             with qubits moved to other address bits     */
 } qsv_program_info;
 
 int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,

@@ -82,6 +82,7 @@ class QsvProgramInfo(C.Structure):
         ("alg_bytes", C.c_double),
         ("flops", C.c_double),
         ("ntiled_phases", C.c_int64),
+        ("ntiled_relabel", C.c_int64),
     ]
 
 

@@ -82,7 +82,7 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
     // brickwork, 10% faster; complex128 stays at 2^12 (a 2^13 tile would
     // leave one block per CU, measured 3% slower despite 15 passes for 17)
     const int tb = jit ? jit_tile_bits(dtype, n) : kQTileBits;
-    QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt, tb));
+    QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt, tb, jit));
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     qt_pass_off = 0;
     qt_phase_off = up(qt.passes.size() * sizeof(QPass));
@@ -196,6 +196,8 @@ void Program::info(qsv_program_info* out) const {
     const int eb = dtype == QSV_C128 ? 16 : 8;
     out->nlaunch = int64_t(qt.passes.size());
     out->ntiled_phases = int64_t(qt.phases.size());
+    for (const QPass& P : qt.passes)
+      out->ntiled_relabel += std::memcmp(P.tbits, P.tbits_out, sizeof(P.tbits)) != 0 ? 1 : 0;
     out->alg_bytes = double(qt.passes.size()) * 2.0 * std::ldexp(1.0, n) * eb;
     return;
   }

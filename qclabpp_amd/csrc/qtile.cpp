@@ -10,6 +10,8 @@ namespace qsv {
 
 namespace {
 
+constexpr int kLaneBits = 6;  // a wavefront's 64 lanes
+
 bool is_one_qubit(const KOp& op) {
   return op.kind == K_DENSE1 || op.kind == K_DIAG1 || op.kind == K_PHASE1;
 }
@@ -262,7 +264,7 @@ namespace {
 // absorb equally many pending gates: strategy 0 takes the highest address
 // bit, 1 the lowest, others a pseudo-random one (seeded by the strategy).
 int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, int strategy,
-                       QProgram& out) {
+                       bool relabel, QProgram& out) {
   constexpr int RB = kQRegBits, NR = 1 << kQRegBits;
   uint64_t rng = 0x9E3779B97F4A7C15ull * uint64_t(strategy + 1);
   auto next = [&]() {
@@ -277,7 +279,15 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
   out.gates_absorbed = in.size();
   const size_t window = 512;  // ops considered when choosing a tile
   std::vector<char> inS(n), blocked(n);
-  while (!rem.empty()) {
+  // qubit <-> address bit (MSB order at the start and at the end)
+  std::vector<int> lay(n), at(n);
+  for (int q = 0; q < n; ++q) {
+    lay[q] = n - 1 - q;
+    at[n - 1 - q] = q;
+  }
+  bool restore = false;  // the pass being formed only undoes the relabeling
+  int nrestore = 0;
+  while (!rem.empty() || restore) {
     // ---- tile qubits: the line bits, the first pending op, then greedily the
     // qubit that lets the tile absorb the most pending ops
     std::fill(inS.begin(), inS.end(), 0);
@@ -288,17 +298,30 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
         ++size;
       }
     };
-    for (int j = 0; j < line_bits && j < n; ++j) add(n - 1 - j);
-    for (int j = 0; j < rem[0].nq; ++j) add(rem[0].q[j]);
+    for (int j = 0; j < line_bits && j < n; ++j) add(at[j]);
+    if (restore) {
+      // displaced qubits along their cycles (q, the qubit on q's home, ...),
+      // so that the tile holds the home address of each but the last
+      for (int q = 0; q < n && size < T; ++q)
+        for (int x = q; lay[x] != n - 1 - x && !inS[x] && size < T; x = at[n - 1 - x]) add(x);
+      for (int b = 0; b < n && size < T; ++b) add(at[b]);
+    } else {
+      for (int j = 0; j < rem[0].nq; ++j) add(rem[0].q[j]);
+    }
     while (size < T) {
       std::set<int> cand;
       for (size_t i = 0; i < std::min(window, rem.size()); ++i)
         for (int j = 0; j < rem[i].nq; ++j)
           if (!inS[rem[i].q[j]]) cand.insert(rem[i].q[j]);
       if (cand.empty()) {
-        // no pending gate needs another qubit: take the lowest address bits,
-        // which lengthens the contiguous runs each tile reads and writes
-        for (int q = n - 1; q >= 0 && size < T; --q) add(q);
+        // no pending gate needs another qubit: with relabeling, displaced
+        // qubits along their cycles (so this pass can send them home), then
+        // the lowest address bits, which lengthen the contiguous runs each
+        // tile reads and writes
+        if (relabel)
+          for (int q = 0; q < n && size < T; ++q)
+            for (int x = q; lay[x] != n - 1 - x && !inS[x] && size < T; x = at[n - 1 - x]) add(x);
+        for (int b = 0; b < n && size < T; ++b) add(at[b]);
         break;
       }
       int best = -1, bq = -1;
@@ -327,16 +350,95 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
         deferred.push_back(op);
       }
     }
-    if (mine.empty()) return fail(QSV_EINVAL, "tiled compile made no progress");
-    // ---- pass descriptor
+    if (mine.empty() && !restore) return fail(QSV_EINVAL, "tiled compile made no progress");
+    // ---- the layout this pass writes: qubits the pending gates touch first
+    // move onto the line bits (their old addresses take the qubits evicted
+    // from the line); the last pass sends every qubit home when it can, and
+    // otherwise one more pass (a restore pass) does
+    std::vector<int> lay2 = lay;
+    if (relabel) {
+      // the line addresses 0..L-1 take first the tile qubits the pending
+      // gates touch soonest, then qubits whose home they are, then the
+      // qubits already there; every other tile qubit goes home when its
+      // home address is in the tile, else keeps its address where it can
+      // At most kLaneBits - L qubits join the line per pass: the tile-local
+      // bits of the lines read (L) and written then fit in the 6 address
+      // bits a wave's 64 lanes span, so both sides move whole lines.
+      std::vector<int> nl;
+      int fresh = 0;
+      const int max_fresh = kLaneBits - line_bits;
+      auto in_nl = [&](int q) { return std::find(nl.begin(), nl.end(), q) != nl.end(); };
+      auto take = [&](int q) {
+        if (!inS[q] || in_nl(q) || int(nl.size()) >= line_bits) return;
+        if (lay[q] >= line_bits) {
+          if (fresh >= max_fresh) return;
+          ++fresh;
+        }
+        nl.push_back(q);
+      };
+      for (const KOp& op : deferred) {
+        for (int j = 0; j < op.nq; ++j) take(op.q[j]);
+        if (int(nl.size()) >= line_bits) break;
+      }
+      for (int j = 0; j < line_bits; ++j) take(n - 1 - j);
+      for (int j = 0; j < line_bits; ++j) take(at[j]);
+      std::vector<int> addr;  // the tile's address bits
+      for (int q = 0; q < n; ++q)
+        if (inS[q]) addr.push_back(lay[q]);
+      std::sort(addr.begin(), addr.end());
+      std::vector<char> used(n, 0), done(n, 0);
+      auto put = [&](int q, int b) {
+        lay2[q] = b;
+        used[b] = 1;
+        done[q] = 1;
+      };
+      auto in_tile = [&](int b) { return std::binary_search(addr.begin(), addr.end(), b); };
+      // line qubits: home, else current, else the first free line address
+      for (int q : nl)
+        if (n - 1 - q < line_bits) put(q, n - 1 - q);
+      for (int q : nl)
+        if (!done[q] && lay[q] < line_bits && !used[lay[q]]) put(q, lay[q]);
+      for (int q : nl)
+        for (int b = 0; b < line_bits && !done[q]; ++b)
+          if (!used[b]) put(q, b);
+      // the rest: home, else current, else the lowest free tile address
+      for (int q = 0; q < n; ++q)
+        if (inS[q] && !done[q] && n - 1 - q >= line_bits && in_tile(n - 1 - q) && !used[n - 1 - q])
+          put(q, n - 1 - q);
+      for (int q = 0; q < n; ++q)
+        if (inS[q] && !done[q] && lay[q] >= line_bits && !used[lay[q]]) put(q, lay[q]);
+      for (int q = 0; q < n; ++q)
+        for (size_t i = 0; i < addr.size() && inS[q] && !done[q]; ++i)
+          if (addr[i] >= line_bits && !used[addr[i]]) put(q, addr[i]);
+    }
+    // ---- pass descriptor; tile-local bits: the line bits read, then the
+    // line bits written, then the rest by address
     QPass P{};
-    std::vector<int> tb;  // tile address bits ascending
-    for (int q = 0; q < n; ++q)
-      if (inS[q]) tb.push_back(n - 1 - q);
-    std::sort(tb.begin(), tb.end());
-    for (int i = 0; i < T; ++i) P.tbits[i] = uint8_t(tb[i]);
-    uint64_t tmask = 0;
-    for (int b : tb) tmask |= uint64_t(1) << b;
+    std::vector<int> lq;  // qubit of tile-local bit i
+    std::vector<int> at2(n);
+    for (int q = 0; q < n; ++q) at2[lay2[q]] = q;
+    for (int j = 0; j < line_bits; ++j) lq.push_back(at[j]);
+    for (int j = 0; j < line_bits; ++j)
+      if (std::find(lq.begin(), lq.end(), at2[j]) == lq.end()) lq.push_back(at2[j]);
+    for (int b = 0; b < n; ++b)
+      if (inS[at[b]] && std::find(lq.begin(), lq.end(), at[b]) == lq.end()) lq.push_back(at[b]);
+    if (int(lq.size()) != T) return fail(QSV_EINVAL, "tiled compile: inconsistent tile relabeling");
+    std::vector<int> loc(n, -1);
+    uint64_t tmask = 0, tmask_out = 0;
+    for (int i = 0; i < T; ++i) {
+      loc[lq[i]] = i;
+      P.tbits[i] = uint8_t(lay[lq[i]]);
+      P.tbits_out[i] = uint8_t(lay2[lq[i]]);
+      tmask |= uint64_t(1) << lay[lq[i]];
+      tmask_out |= uint64_t(1) << lay2[lq[i]];
+    }
+    // the pass writes exactly the addresses it reads (in place), and the
+    // lanes of a wave cover whole lines on both sides
+    if (tmask != tmask_out || __builtin_popcountll(tmask) != T)
+      return fail(QSV_EINVAL, "tiled compile: a pass would write outside its tile");
+    for (int i = 0; i < T; ++i)
+      if ((i < line_bits && P.tbits[i] != i) || (P.tbits_out[i] < line_bits && i >= kLaneBits))
+        return fail(QSV_EINVAL, "tiled compile: line bits outside the lanes of a wave");
     P.nruns = 0;
     for (int b = 0; b < n;) {
       if (tmask >> b & 1) {
@@ -351,9 +453,7 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
       b = e;
     }
     P.ntiles = uint64_t(1) << (n - T);
-    auto local = [&](int q) {  // tile-local bit of qubit q
-      return int(std::lower_bound(tb.begin(), tb.end(), n - 1 - q) - tb.begin());
-    };
+    auto local = [&](int q) { return loc[q]; };  // tile-local bit of qubit q
     // ---- phases
     P.phase_begin = int(out.phases.size());
     // phase grouping: the fusion-style one or the greedy one under a few tie
@@ -363,6 +463,7 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
       std::vector<Group> alt = make_phases_greedy(mine, n, RB, st);
       if (alt.size() < groups.size()) groups.swap(alt);
     }
+    if (groups.empty()) groups.push_back(Group{});  // a restore pass: one phase, no ops
     for (const Group& g : groups) {
       uint32_t regmask = 0;
       for (int q : g.q) regmask |= 1u << local(q);
@@ -535,13 +636,22 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
     P.phase_end = int(out.phases.size());
     out.passes.push_back(P);
     rem.swap(deferred);
+    lay = lay2;
+    for (int q = 0; q < n; ++q) at[lay[q]] = q;
+    nrestore += restore ? 1 : 0;
+    restore = false;
+    if (rem.empty())
+      for (int q = 0; q < n; ++q) restore = restore || lay[q] != n - 1 - q;
+    if (restore && nrestore >= 3)
+      return fail(QSV_EUNSUP, "tiled compile: relabeling displaced too many qubits to restore");
   }
   return QSV_OK;
 }
 
 }  // namespace
 
-int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out, int tile_bits) {
+int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out, int tile_bits,
+                  bool relabel) {
   if (tile_bits < kQTileBits || tile_bits > kQTileBitsMax)
     return fail(QSV_EINVAL, "tile bits must be in [" + std::to_string(kQTileBits) + ", " +
                                 std::to_string(kQTileBitsMax) + "]");
@@ -553,15 +663,21 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
   // The greedy tile choice depends on how ties are broken; a few rules are
   // tried and the schedule with the fewest passes (then phases) is kept:
   // each pass costs a full read and write of the state.
+  // With relabeling allowed, schedules with and without it compete.
   constexpr int kStrategies = 8;
-  for (int st = 0; st < kStrategies; ++st) {
-    QProgram cand;
-    QSV_TRY(compile_qtile_with(in, n, line_bits, tile_bits, st, cand));
-    cand.tile_bits = tile_bits;
-    if (st == 0 || cand.passes.size() < out.passes.size() ||
-        (cand.passes.size() == out.passes.size() && cand.phases.size() < out.phases.size()))
-      out = std::move(cand);
-  }
+  bool have = false;
+  for (int rl = 0; rl <= (relabel ? 1 : 0); ++rl)
+    for (int st = 0; st < kStrategies; ++st) {
+      QProgram cand;
+      const int rc = compile_qtile_with(in, n, line_bits, tile_bits, st, rl == 1, cand);
+      if (rc == QSV_EUNSUP && rl == 1) continue;  // too many qubits displaced to restore
+      QSV_TRY(rc);
+      cand.tile_bits = tile_bits;
+      if (!have || cand.passes.size() < out.passes.size() ||
+          (cand.passes.size() == out.passes.size() && cand.phases.size() < out.phases.size()))
+        out = std::move(cand);
+      have = true;
+    }
   return QSV_OK;
 }
 

@@ -67,11 +67,21 @@ inline __host__ __device__ uint32_t qtile_swizzle(uint32_t p) {
   return p ^ (((p >> 4) ^ (p >> 8)) & 15u);
 }
 
+// A pass reads its tile with tile-local bit i at address bit tbits[i] and
+// writes it back with bit i at tbits_out[i].  Both are the same set of
+// address bits (the pass runs in place); tbits_out permutes them when the
+// pass moves other qubits onto the line bits for the passes after it (a
+// relabeling of qubits to address bits, undone by the end of the program).
+// Local bits 0..L-1 are the line bits read (address bits 0..L-1), and the
+// line bits written are among local bits 0..2L-1, so the lanes of a wave
+// cover whole 128-byte lines on both sides.  Without relabeling the local
+// bits are the address bits in ascending order and tbits_out == tbits.
 struct QPass {
   int32_t phase_begin, phase_end;
   int32_t nruns;
-  uint8_t tbits[16];    // global address bit of tile-local bit i, ascending
-  uint8_t rdst[24];     // runs of address bits outside the tile
+  uint8_t tbits[16];      // address bit of tile-local bit i when read
+  uint8_t tbits_out[16];  // address bit of tile-local bit i when written
+  uint8_t rdst[24];       // runs of address bits outside the tile
   uint8_t rlen[24];
   uint64_t ntiles;
 };
@@ -88,9 +98,11 @@ struct QProgram {
 // Compile lowered, unfused gates (n >= tile_bits) into passes and phases.
 // line_bits: address bits inside one 128-byte line, always kept in the tile.
 // tile_bits: address bits per tile (kQTileBits for the interpreting kernel,
-// up to kQTileBitsMax for compiled passes).
+// up to kQTileBitsMax for compiled passes).  relabel: passes may move qubits
+// onto the line bits (tbits_out != tbits; compiled passes only), so a tile
+// need not spend its line bits on qubits the next gates do not touch.
 int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out,
-                  int tile_bits = kQTileBits);
+                  int tile_bits = kQTileBits, bool relabel = false);
 
 struct QTileLaunch {
   void* psi;

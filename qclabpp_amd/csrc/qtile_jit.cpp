@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <map>
 #include <mutex>
+#include <cstring>
 #include <sstream>
 #include <thread>
 
@@ -151,27 +152,37 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128) {
 // 64-bit literal
 std::string u64(uint64_t x) { return std::to_string(x) + "ull"; }
 
-// Global offset (in amplitudes) of tile position p of pass P.
-uint64_t gofs(const QPass& P, int T, uint32_t p) {
+// Global offset (in amplitudes) of tile position p under the address bits
+// tb of the tile-local bits (the pass's tbits when reading, tbits_out when
+// writing).
+uint64_t gofs(const uint8_t* tb, int T, uint32_t p) {
   uint64_t o = 0;
   for (int b = 0; b < T; ++b)
-    if (p >> b & 1u) o |= uint64_t(1) << P.tbits[b];
+    if (p >> b & 1u) o |= uint64_t(1) << tb[b];
   return o;
 }
 
 // Expression for the global offset of this thread's share of a phase:
 // thread index bit j sits on the j-th lowest tile bit outside the phase's
 // registers.
-std::string thread_gofs(const QPass& P, int T, uint32_t regmask) {
+std::string thread_gofs(const uint8_t* tb, int T, uint32_t regmask) {
   std::ostringstream e;
   e << "(0ull";
   for (int b = 0, j = 0; b < T && j < T - kQRegBits; ++b)
     if (!(regmask >> b & 1u)) {
-      e << " | ((unsigned long long)((tid >> " << j << ") & 1u) << " << int(P.tbits[b]) << ")";
+      e << " | ((unsigned long long)((tid >> " << j << ") & 1u) << " << int(tb[b]) << ")";
       ++j;
     }
   e << ")";
   return e.str();
+}
+
+// Tile-local bits whose address bit lies inside a 128-byte line.
+uint32_t line_mask(const uint8_t* tb, int T, bool c128) {
+  uint32_t m = 0;
+  for (int b = 0; b < T; ++b)
+    if (tb[b] < (c128 ? 3 : 4)) m |= 1u << b;
+  return m;
 }
 
 // A pass as one kernel.  Each tile goes HBM -> phases -> HBM.  A phase
@@ -189,9 +200,18 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
   const int K = P.phase_end - P.phase_begin;
   const QPhase& F = qp.phases[P.phase_begin];
   const QPhase& L = qp.phases[P.phase_end - 1];
-  const uint32_t line = c128 ? 7u : 15u;  // tile bits of one 128-byte line
-  const bool din = !(F.regmask & line);
-  const bool dout = !(L.regmask & line);
+  // tile-local bits of the 128-byte lines read and written (the same ones
+  // unless the pass relabels qubits)
+  const uint32_t line_in = line_mask(P.tbits, T, c128), line_out = line_mask(P.tbits_out, T, c128);
+  const bool din = !(F.regmask & line_in);
+  // A pass that relabels stores each amplitude to another address of its
+  // tile than the one it came from, so no store may start before every
+  // load of the tile has landed.  Passes of two or more phases guarantee
+  // that with the barrier after their first scatter into LDS (which needs
+  // the loaded values); a one-phase pass with both ends direct has no
+  // barrier, so it stores through LDS instead.
+  const bool relabels = std::memcmp(P.tbits, P.tbits_out, sizeof(P.tbits)) != 0;
+  const bool dout = !(L.regmask & line_out) && !(K == 1 && din && relabels);
   // waves per SIMD the register budget is sized for, from the blocks per CU
   // that LDS allows (160 KiB): complex128 tiles of 2^12 (64 KiB, 2 blocks of
   // 4 waves), complex64 tiles of 2^12 (32 KiB, 3 blocks of 4 waves) or 2^13
@@ -214,11 +234,13 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
   // the first (last) phase's registers, or tile positions tid + threads*j
   std::vector<uint64_t> in_ofs(NR), out_ofs(NR);
   for (int i = 0; i < NR; ++i) {
-    in_ofs[i] = din ? gofs(P, T, qtile_swizzle(F.regpos[i])) : gofs(P, T, threads * uint32_t(i));
-    out_ofs[i] = dout ? gofs(P, T, qtile_swizzle(L.regpos_out[i])) : gofs(P, T, threads * uint32_t(i));
+    in_ofs[i] = din ? gofs(P.tbits, T, qtile_swizzle(F.regpos[i])) : gofs(P.tbits, T, threads * uint32_t(i));
+    out_ofs[i] = dout ? gofs(P.tbits_out, T, qtile_swizzle(L.regpos_out[i]))
+                      : gofs(P.tbits_out, T, threads * uint32_t(i));
   }
-  s << "  const unsigned long long gin = " << thread_gofs(P, T, din ? F.regmask : tile_order) << ";\n"
-    << "  const unsigned long long gout = " << thread_gofs(P, T, dout ? L.regmask : tile_order) << ";\n";
+  s << "  const unsigned long long gin = " << thread_gofs(P.tbits, T, din ? F.regmask : tile_order) << ";\n"
+    << "  const unsigned long long gout = " << thread_gofs(P.tbits_out, T, dout ? L.regmask : tile_order)
+    << ";\n";
   auto arr = [&](const char* name, const std::vector<uint64_t>& v) {
     s << "  const unsigned long long " << name << "[16] = {";
     for (int i = 0; i < NR; ++i) s << (i ? ", " : "") << u64(v[i]);

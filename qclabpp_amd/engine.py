@@ -181,7 +181,8 @@ class Program:
         check(lib().qsv_program_info_get(self._h, C.byref(inf)))
         return {"ngates": inf.ngates, "nlaunch": inf.nlaunch, "nplain": inf.nplain,
                 "ndiag": inf.ndiag, "ndense": list(inf.ndense), "alg_bytes": inf.alg_bytes,
-                "flops": inf.flops, "tiled_phases": inf.ntiled_phases}
+                "flops": inf.flops, "tiled_phases": inf.ntiled_phases,
+                "tiled_relabel": inf.ntiled_relabel}
 
     def launches(self):
         """Every launch of one run, in order: (qubits, diag, coefficients) with

@@ -145,3 +145,21 @@ def test_jit_load_store_routes(name, dt):
     x = psi if dt == "c128" else psi.astype(np.complex64)
     got = run(E.Program(n, circ, fuse_k=E.TILED_JIT, dtype=dt), x)
     assert np.abs(got - want).max() < (1e-12 if dt == "c128" else 5e-5)
+
+
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+@pytest.mark.parametrize("n", [18, 20, 22])
+def test_jit_relabeling_passes(n, dt):
+    """Compiled passes that move qubits onto the line bits (and back by the
+    end of the program): brickwork plus every gate kind, against the
+    oracle, with at least one relabeling pass in the schedule."""
+    circ = W.brickwork(n, 20, seed=30 + n) + every_kind(n, n) + W.brickwork(n, 6, seed=n)
+    psi = random_state(n, 40 + n, np.complex128)
+    want = oracle.apply(n, circ, psi)
+    prog = E.Program(n, circ, fuse_k=E.TILED_JIT, dtype=dt)
+    info = prog.info()
+    print(f"relabel n={n} {dt}: passes={info['nlaunch']} relabeling={info['tiled_relabel']}", flush=True)
+    assert info["tiled_relabel"] > 0
+    x = psi if dt == "c128" else psi.astype(np.complex64)
+    got = run(prog, x)
+    assert np.abs(got - want).max() < (1e-12 if dt == "c128" else 5e-5)

@@ -45,3 +45,13 @@ def test_bad_arguments_are_rejected():
     lib = _lib.lib()
     assert lib.qsv_tiled_jit_compile_check(8, _lib.QSV_C128, None, 0, None, None) == _lib.QSV_EINVAL
     assert lib.qsv_tiled_jit_compile_check(14, 7, None, 0, None, None) == _lib.QSV_EDTYPE
+
+
+@pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
+def test_relabeling_schedules_compile(dtype):
+    """Schedules whose passes move qubits onto the line bits: the scheduler
+    checks that every pass writes exactly its own tile's addresses and that
+    the lines read and written stay within a wave's lanes, then every pass
+    compiles."""
+    n = 20
+    assert _check(n, W.brickwork(n, 20, seed=50) + every_kind(n, 20), dtype) >= 2
