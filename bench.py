@@ -76,7 +76,7 @@ def dominant_programs(E, G, n):
 
 def kernels_only(n):
     """Launch pattern for counter collection: 2 x the compiled tiled brickwork
-    program (2 x 17 pass kernels qtp_*), 2 x the interpreted one (k_qtile),
+    program (its pass kernels qtp_*), 2 x the interpreted one (k_qtile),
     5 x k=5 block, 5 x dense 1-qubit sweep, 3 x copy of 2^n complex128
     (2^n * 16 bytes read and written)."""
     import torch
