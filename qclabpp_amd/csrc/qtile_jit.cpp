@@ -162,17 +162,66 @@ uint64_t gofs(const uint8_t* tb, int T, uint32_t p) {
   return o;
 }
 
-// Expression for the global offset of this thread's share of a phase:
-// thread index bit j sits on the j-th lowest tile bit outside the phase's
-// registers.
-std::string thread_gofs(const uint8_t* tb, int T, uint32_t regmask) {
+constexpr int kLaneBitsJit = 6;  // a wavefront's 64 lanes
+
+// Tile-local bit of thread index bit j in a phase (ord[j]): the bits outside
+// the phase's registers, in an order chosen per phase.
+//  - `first` (the lines a direct first phase loads) leads, so 8 or 16
+//    consecutive lanes read one 128-byte line;
+//  - `lanes` (the lines a direct last phase stores) must sit among the 6
+//    lane bits, so each line is written by one instruction;
+//  - lane bits 0-2 take bits 0-2 of the slot swizzle's low nibble (local
+//    bit mod 4, for local bits below 12) and lane bit 3 takes bit 3.  Measured
+//    on gfx950: ds_read_b128 is conflict-free once lane bits 0-3 cover the
+//    four nibble bits in any order, ds_write_b128 only when lane bits 0-2
+//    cover nibble bits 0-2 (7.5 conflict cycles per write otherwise).  Lanes
+//    in ascending bit order collided whenever the registers held low local
+//    bits, as relabeled passes often do.
+std::vector<int> lane_order(int T, uint32_t regmask, uint32_t first, uint32_t lanes) {
+  std::vector<int> ord;
+  auto free_bit = [&](int b) {
+    return !(regmask >> b & 1u) && std::find(ord.begin(), ord.end(), b) == ord.end();
+  };
+  auto nib = [](int b) { return b < 12 ? b % 4 : -1; };
+  auto nib_used = [&](int r) {
+    for (size_t i = 0; i < ord.size() && i < 4; ++i)
+      if (nib(ord[i]) == r) return true;
+    return false;
+  };
+  for (int b = 0; b < T; ++b)
+    if ((first >> b & 1u) && free_bit(b)) ord.push_back(b);
+  while (ord.size() < 4) {
+    // wanted nibble bits: an unused one of 0-2 for lane bits 0-2, 3 for lane bit 3
+    auto wanted = [&](int b) {
+      const int r = nib(b);
+      if (r < 0 || nib_used(r)) return false;
+      return ord.size() < 3 ? r < 3 : r == 3;
+    };
+    int pick = -1;
+    for (int pass = 0; pass < 4 && pick < 0; ++pass)
+      for (int b = 0; b < T && pick < 0; ++b) {
+        if (!free_bit(b)) continue;
+        const bool fresh = nib(b) >= 0 && !nib_used(nib(b));
+        if ((pass == 0 && wanted(b) && (lanes >> b & 1u)) || (pass == 1 && wanted(b)) || (pass == 2 && fresh) ||
+            pass == 3)
+          pick = b;
+      }
+    if (pick < 0) break;
+    ord.push_back(pick);
+  }
+  for (int b = 0; b < T; ++b)
+    if ((lanes >> b & 1u) && free_bit(b)) ord.push_back(b);
+  for (int b = 0; b < T; ++b)
+    if (free_bit(b)) ord.push_back(b);
+  return ord;
+}
+
+// Expression for the global offset of this thread's share of a phase.
+std::string thread_gofs(const uint8_t* tb, const std::vector<int>& ord) {
   std::ostringstream e;
   e << "(0ull";
-  for (int b = 0, j = 0; b < T && j < T - kQRegBits; ++b)
-    if (!(regmask >> b & 1u)) {
-      e << " | ((unsigned long long)((tid >> " << j << ") & 1u) << " << int(tb[b]) << ")";
-      ++j;
-    }
+  for (size_t j = 0; j < ord.size(); ++j)
+    e << " | ((unsigned long long)((tid >> " << j << ") & 1u) << " << int(tb[ord[j]]) << ")";
   e << ")";
   return e.str();
 }
@@ -211,7 +260,21 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
   // the loaded values); a one-phase pass with both ends direct has no
   // barrier, so it stores through LDS instead.
   const bool relabels = std::memcmp(P.tbits, P.tbits_out, sizeof(P.tbits)) != 0;
-  const bool dout = !(L.regmask & line_out) && !(K == 1 && din && relabels);
+  bool dout = !(L.regmask & line_out) && !(K == 1 && din && relabels);
+  // per-phase thread orders (the first and last phases' also give the
+  // direct load and store offsets); loads and stores through LDS use
+  // positions tid + threads*j, whose lanes are local bits 0..5
+  std::vector<std::vector<int>> ords(K);
+  for (int k = 0; k < K; ++k) {
+    const QPhase& Q = qp.phases[P.phase_begin + k];
+    ords[k] = lane_order(T, Q.regmask, k == 0 && din ? line_in : 0u, k == K - 1 && dout ? line_out : 0u);
+  }
+  // a direct store needs every written line bit among the 6 lane bits
+  // (always the case for the schedules built here; else store through LDS)
+  if (dout)
+    for (int j = kLaneBitsJit; j < int(ords[K - 1].size()); ++j)
+      if (line_out >> ords[K - 1][j] & 1u) dout = false;
+  if (!dout) ords[K - 1] = lane_order(T, L.regmask, K == 1 && din ? line_in : 0u, 0u);
   // waves per SIMD the register budget is sized for, from the blocks per CU
   // that LDS allows (160 KiB): complex128 tiles of 2^12 (64 KiB, 2 blocks of
   // 4 waves), complex64 tiles of 2^12 (32 KiB, 3 blocks of 4 waves) or 2^13
@@ -238,8 +301,9 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
     out_ofs[i] = dout ? gofs(P.tbits_out, T, qtile_swizzle(L.regpos_out[i]))
                       : gofs(P.tbits_out, T, threads * uint32_t(i));
   }
-  s << "  const unsigned long long gin = " << thread_gofs(P.tbits, T, din ? F.regmask : tile_order) << ";\n"
-    << "  const unsigned long long gout = " << thread_gofs(P.tbits_out, T, dout ? L.regmask : tile_order)
+  const std::vector<int> ord_tile = lane_order(T, tile_order, 0u, 0u);
+  s << "  const unsigned long long gin = " << thread_gofs(P.tbits, din ? ords[0] : ord_tile) << ";\n"
+    << "  const unsigned long long gout = " << thread_gofs(P.tbits_out, dout ? ords[K - 1] : ord_tile)
     << ";\n";
   auto arr = [&](const char* name, const std::vector<uint64_t>& v) {
     s << "  const unsigned long long " << name << "[16] = {";
@@ -292,13 +356,13 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
       << "      const QC R* cf = cf0;\n"
       << "      asm volatile(\"\" : \"+s\"(cf));\n";
     if (gather || scatter) {
-      s << "      unsigned x = tid;\n"
-        << "      asm volatile(\"\" : \"+v\"(x));\n";
-      // spread the thread index over the tile bits outside the registers:
-      // insert a zero at each register bit, lowest first
-      for (int b = 0; b < T; ++b)
-        if (Q.regmask >> b & 1u)
-          s << "      x = ((x >> " << b << ") << " << (b + 1) << ") | (x & " << ((1u << b) - 1) << "u);\n";
+      s << "      unsigned y = tid;\n"
+        << "      asm volatile(\"\" : \"+v\"(y));\n"
+        << "      const unsigned x = 0u";
+      // thread index bit j -> tile-local bit ords[k][j]
+      for (size_t j = 0; j < ords[k].size(); ++j)
+        s << " | (((y >> " << j << ") & 1u) << " << ords[k][j] << ")";
+      s << ";\n";
       s << "      const unsigned tp = swz(x);\n";
     }
     if (gather) {
