@@ -1,5 +1,6 @@
 """Full-size states checked through size-independent properties: the
-30-qubit complex128 brickwork and its inverse return |0...0>; a 33-qubit
+30-qubit complex128 brickwork and its inverse return |0...0> in every
+compiled form, as does the 33-qubit complex64 one through compiled passes; a 33-qubit
 complex64 state (64 GiB, sweeps split over several launches) follows closed
 forms after Hadamard layers, diagonal gates, merged diagonals and Pauli
 products, with FP64 norms and expectation values."""
@@ -23,13 +24,15 @@ def _free():
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("fuse_k,depth", [(5, 20), (4, 20), (0, 3)])
+@pytest.mark.parametrize("fuse_k,depth", [(5, 20), (4, 20), (0, 3), (E.TILED, 20), (E.TILED_JIT, 20)])
 def test_brickwork30_c128_inverse_returns_to_zero(fuse_k, depth):
     import torch
     n = 30
     circ = W.brickwork(n, depth)
     fwd = E.Program(n, circ, fuse_k=fuse_k)
     back = E.Program(n, G.inverse(circ), fuse_k=fuse_k)
+    if fuse_k == E.TILED_JIT:  # compiled passes that relabel qubits, at full size
+        assert fwd.info()["tiled_relabel"] > 0 and back.info()["tiled_relabel"] > 0
     psi = E.new_state(n, torch.complex128)
     fwd.run(psi)
     torch.cuda.synchronize()
@@ -39,6 +42,28 @@ def test_brickwork30_c128_inverse_returns_to_zero(fuse_k, depth):
     torch.cuda.synchronize()
     assert abs(psi[0].item() - 1.0) < 1e-11
     assert abs(O.norm2(psi) - 1.0) < 1e-12
+    del psi
+    _free()
+
+
+def test_brickwork33_c64_compiled_inverse_returns_to_zero():
+    """64 GiB complex64 state through relabeling compiled passes (2^13 tiles)
+    and back; FP64 norms."""
+    import torch
+    n = 33
+    circ = W.brickwork(n, 20, seed=W.SEED_PREC)
+    fwd = E.Program(n, circ, fuse_k=E.TILED_JIT, dtype="c64")
+    back = E.Program(n, G.inverse(circ), fuse_k=E.TILED_JIT, dtype="c64")
+    assert fwd.info()["tiled_relabel"] > 0
+    psi = E.new_state(n, torch.complex64)
+    fwd.run(psi)
+    torch.cuda.synchronize()
+    assert abs(psi[0].item()) < 0.5
+    assert abs(O.norm2(psi) - 1.0) < 1e-5
+    back.run(psi)
+    torch.cuda.synchronize()
+    assert abs(psi[0].item() - 1.0) < 1e-4
+    assert abs(O.norm2(psi) - 1.0) < 1e-5
     del psi
     _free()
 
