@@ -9,9 +9,14 @@
 // States are recomputed by inversion instead of being stored: the sweep needs
 // two state buffers whatever the depth.
 //
+// A parametrised gate's step -- the inner product and both un-applies -- is
+// one pass over the two states (k_adjoint_step): 4 S of HBM traffic instead
+// of 6 S for an inner-product pass plus two sweeps.
+//
 // Rotations exp(-i theta P / 2) give dE/dtheta = Im<lam|P|phi>; Phase/CPhase
 // diag(1, e^{i theta}) give dE/dtheta = -2 Im<lam|Pi_1|phi> (Pi_1 projects the
 // target, and the control, onto the phased state).
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -110,6 +115,57 @@ bool generator(const qsv_gate& g, int n, PauliOp& G, double& wr, double& wi) {
       return false;
   }
   return true;
+}
+
+// The fused reverse-sweep step of a parametrised gate: the inverse gate as a
+// 2^K x 2^K matrix on its own qubits (built on the host by applying the
+// lowered op to basis states) and the generator's partner / weight tables.
+// The generator acts on the gate's qubits only (flip, zmask and pmask lie
+// inside them), so both come from the same group of amplitudes.
+int adjoint_step(const KOp& inv, const PauliOp& G, int n, AdjStep& out) {
+  out = AdjStep();
+  std::vector<int> q(inv.q, inv.q + inv.nq);
+  std::sort(q.begin(), q.end());
+  const int K = int(q.size());
+  if (K < 1 || K > 2) return fail(QSV_EUNSUP, "adjoint step: gate on more than 2 qubits");
+  out.K = K;
+  const int D = 1 << K;
+  uint64_t gate_mask = 0;
+  for (int j = 0; j < K; ++j) gate_mask |= uint64_t(1) << (n - 1 - q[j]);
+  if ((G.flip | G.zmask | G.pmask) & ~gate_mask)
+    return fail(QSV_EINVAL, "adjoint step: generator outside the gate's qubits");
+  // address bits ascending (= qubits descending)
+  for (int j = 0; j < K; ++j) out.bits[j] = n - 1 - q[K - 1 - j];
+  auto addr = [&](int c) {  // local c -> address offset; bit K-1-j of c is qubit q[j]
+    uint64_t o = 0;
+    for (int j = 0; j < K; ++j)
+      if (c >> (K - 1 - j) & 1) o |= uint64_t(1) << (n - 1 - q[j]);
+    return o;
+  };
+  for (int c = 0; c < D; ++c) {
+    const uint64_t y = addr(c);
+    out.dep[c] = y;
+    int pc = 0;
+    for (int e = 0; e < D; ++e)
+      if (addr(e) == (y ^ G.flip)) pc = e;
+    out.partner[c] = pc;
+    const bool on = (y & G.pmask) == G.pval;
+    out.w[c] = on ? ((__builtin_popcountll(y & G.zmask) & 1) ? -1.0 : 1.0) : 0.0;
+  }
+  KOp local = inv;  // renumber to the gate's own qubits
+  for (int j = 0; j < inv.nq; ++j)
+    local.q[j] = int(std::lower_bound(q.begin(), q.end(), inv.q[j]) - q.begin());
+  std::vector<cd> col(D);
+  for (int c = 0; c < D; ++c) {
+    std::fill(col.begin(), col.end(), cd(0.0, 0.0));
+    col[c] = 1.0;
+    apply_op_host(local, K, col.data());
+    for (int r = 0; r < D; ++r) {
+      out.m[2 * (r * D + c)] = col[r].real();
+      out.m[2 * (r * D + c) + 1] = col[r].imag();
+    }
+  }
+  return QSV_OK;
 }
 
 template <typename R>
@@ -281,12 +337,18 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
     if (gs[kk].param >= 0) {
       if (!generator(gs[kk], n, G, a, b))
         return fail(QSV_EUNSUP, "gate kind " + std::to_string(gs[kk].kind) + " is not differentiable");
-      QSV_TRY(inner_any(dtype, lam, psi, n, G, partials + size_t(rows) * kRedBlocks, st));
+      // <lam|G|psi> and the un-apply on both states in one pass over them
+      AdjStep step;
+      QSV_TRY(adjoint_step(inv[kk], G, n, step));
+      QSV_TRY(dtype == QSV_C128
+                  ? launch_adjoint_step<double>(psi, lam, n, step, partials + size_t(rows) * kRedBlocks, st)
+                  : launch_adjoint_step<float>(psi, lam, n, step, partials + size_t(rows) * kRedBlocks, st));
       // fold the generator's own phase into the weights: z_true = phase * z
       row_param.push_back(gs[kk].param);
       row_wr.push_back(a * G.phase_re + b * G.phase_im);
       row_wi.push_back(-a * G.phase_im + b * G.phase_re);
       ++rows;
+      continue;
     }
     QSV_TRY(plain_any(dtype, inv[kk], psi, n, st));
     QSV_TRY(plain_any(dtype, inv[kk], lam, n, st));

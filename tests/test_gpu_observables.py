@@ -128,3 +128,20 @@ def test_autograd_function():
     ps = oracle.param_shift(n, circ, H, theta.detach().numpy(), list(range(theta.numel())))
     assert abs(e.item() - oracle.energy(n, circ, H, theta.detach().numpy())) < 1e-12
     assert np.abs(theta.grad.numpy() - ps).max() < 1e-12
+
+
+def test_adjoint_complex64_vs_complex128():
+    """The fused reverse-sweep step in FP32 arithmetic with FP64 inner
+    products, against the complex128 gradient."""
+    import torch
+    n = 10
+    circ = W.tfim_ansatz(n, 3, seed=9)
+    H = O.tfim(n)
+    out = {}
+    for dt in (torch.complex128, torch.complex64):
+        psi = E.new_state(n, dt)
+        lam = torch.empty_like(psi)
+        out[dt] = O.adjoint_gradient(psi, lam, circ, H, fuse_k=0)
+    (e128, g128), (e64, g64) = out[torch.complex128], out[torch.complex64]
+    assert abs(e64 - e128) < 1e-5
+    assert np.abs(np.asarray(g64) - np.asarray(g128)).max() < 1e-4
