@@ -9,9 +9,11 @@
 // States are recomputed by inversion instead of being stored: the sweep needs
 // two state buffers whatever the depth.
 //
-// A parametrised gate's step -- the inner product and both un-applies -- is
-// one pass over the two states (k_adjoint_step): 4 S of HBM traffic instead
-// of 6 S for an inner-product pass plus two sweeps.
+// The reverse sweep runs as LDS tile passes (k_adjoint_tiles): the inverse
+// gates, last first, are scheduled like a circuit, and within a pass each
+// parametrised gate's inner product is taken over every tile just before the
+// gate's inverse is applied there, so a pass over both states (4 S of HBM
+// traffic) covers many gates.
 //
 // Rotations exp(-i theta P / 2) give dE/dtheta = Im<lam|P|phi>; Phase/CPhase
 // diag(1, e^{i theta}) give dE/dtheta = -2 Im<lam|Pi_1|phi> (Pi_1 projects the
@@ -26,6 +28,7 @@
 #include "observables.hpp"
 #include "plain.hpp"
 #include "program.hpp"
+#include "tile.hpp"
 
 namespace qsv {
 namespace {
@@ -50,6 +53,30 @@ struct Scratch {
     return QSV_OK;
   }
 } g_scratch;
+
+// Tile bits of the adjoint's LDS passes: two tiles (phi and lam) of 32 KiB
+// each, plus the row accumulators, leave room for two blocks per CU.
+constexpr int kAdjTileBits128 = 11;
+constexpr int kAdjTileBits64 = 12;
+
+// A device copy of a host array, freed with the object.
+template <typename T>
+struct DevArray {
+  T* p = nullptr;
+  ~DevArray() {
+    if (p) (void)hipFree(p);
+  }
+  int upload(const std::vector<T>& v) {
+    if (v.empty()) return QSV_OK;
+    if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return fail(QSV_ENOMEM, "device allocation of the adjoint pass tables failed");
+    }
+    QSV_CHECK_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return QSV_OK;
+  }
+};
 
 int check_state(const void* psi, int n, int dtype) {
   if (!psi) return fail(QSV_EINVAL, "null state pointer");
@@ -115,57 +142,6 @@ bool generator(const qsv_gate& g, int n, PauliOp& G, double& wr, double& wi) {
       return false;
   }
   return true;
-}
-
-// The fused reverse-sweep step of a parametrised gate: the inverse gate as a
-// 2^K x 2^K matrix on its own qubits (built on the host by applying the
-// lowered op to basis states) and the generator's partner / weight tables.
-// The generator acts on the gate's qubits only (flip, zmask and pmask lie
-// inside them), so both come from the same group of amplitudes.
-int adjoint_step(const KOp& inv, const PauliOp& G, int n, AdjStep& out) {
-  out = AdjStep();
-  std::vector<int> q(inv.q, inv.q + inv.nq);
-  std::sort(q.begin(), q.end());
-  const int K = int(q.size());
-  if (K < 1 || K > 2) return fail(QSV_EUNSUP, "adjoint step: gate on more than 2 qubits");
-  out.K = K;
-  const int D = 1 << K;
-  uint64_t gate_mask = 0;
-  for (int j = 0; j < K; ++j) gate_mask |= uint64_t(1) << (n - 1 - q[j]);
-  if ((G.flip | G.zmask | G.pmask) & ~gate_mask)
-    return fail(QSV_EINVAL, "adjoint step: generator outside the gate's qubits");
-  // address bits ascending (= qubits descending)
-  for (int j = 0; j < K; ++j) out.bits[j] = n - 1 - q[K - 1 - j];
-  auto addr = [&](int c) {  // local c -> address offset; bit K-1-j of c is qubit q[j]
-    uint64_t o = 0;
-    for (int j = 0; j < K; ++j)
-      if (c >> (K - 1 - j) & 1) o |= uint64_t(1) << (n - 1 - q[j]);
-    return o;
-  };
-  for (int c = 0; c < D; ++c) {
-    const uint64_t y = addr(c);
-    out.dep[c] = y;
-    int pc = 0;
-    for (int e = 0; e < D; ++e)
-      if (addr(e) == (y ^ G.flip)) pc = e;
-    out.partner[c] = pc;
-    const bool on = (y & G.pmask) == G.pval;
-    out.w[c] = on ? ((__builtin_popcountll(y & G.zmask) & 1) ? -1.0 : 1.0) : 0.0;
-  }
-  KOp local = inv;  // renumber to the gate's own qubits
-  for (int j = 0; j < inv.nq; ++j)
-    local.q[j] = int(std::lower_bound(q.begin(), q.end(), inv.q[j]) - q.begin());
-  std::vector<cd> col(D);
-  for (int c = 0; c < D; ++c) {
-    std::fill(col.begin(), col.end(), cd(0.0, 0.0));
-    col[c] = 1.0;
-    apply_op_host(local, K, col.data());
-    for (int r = 0; r < D; ++r) {
-      out.m[2 * (r * D + c)] = col[r].real();
-      out.m[2 * (r * D + c) + 1] = col[r].imag();
-    }
-  }
-  return QSV_OK;
 }
 
 template <typename R>
@@ -312,9 +288,6 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
   // lam = H phi
   QSV_TRY(qsv_dev_apply_pauli_sum(psi, lam, n, dtype, terms, nterms, stream));
   // per-gate generators and un-apply ops
-  std::vector<PauliOp> gens;
-  std::vector<int> gen_gate;
-  std::vector<double> wr, wi;
   std::vector<KOp> inv(gs.size());
   for (size_t k = 0; k < gs.size(); ++k) {
     qsv_gate c = gs[k];
@@ -331,27 +304,80 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
   int rows = 1;
   std::vector<int> row_param(1, -1);
   std::vector<double> row_wr(1, 0), row_wi(1, 0);
-  for (size_t kk = gs.size(); kk-- > 0;) {
-    PauliOp G;
-    double a = 0, b = 0;
-    if (gs[kk].param >= 0) {
+  // The reverse sweep in LDS tile passes: the inverse gates, last first,
+  // scheduled like a batch circuit; within a pass each parametrised op
+  // contributes <lam|G|phi> over every tile just before it is applied.
+  std::vector<KOp> rev(gs.size());
+  for (size_t k = 0; k < gs.size(); ++k) rev[gs.size() - 1 - k] = inv[k];
+  std::vector<TPass> passes;
+  std::vector<TOp> tops;
+  std::vector<int> origin;
+  if (!rev.empty())
+    QSV_TRY(compile_tiled(rev, n, dtype == QSV_C128 ? kAdjTileBits128 : kAdjTileBits64, false, 0, passes, tops,
+                          &origin));
+  std::vector<TGen> gens(tops.size());
+  std::vector<TPass> runs;        // passes split to at most kAdjRowsMax rows each
+  std::vector<int> run_rows;      // rows of each run
+  std::vector<int> run_row0;      // first global row of each run
+  for (const TPass& P : passes) {
+    auto split = [&](uint64_t mask, uint32_t& loc, uint64_t& out) {
+      loc = 0;
+      out = 0;
+      for (int b = 0; b < n; ++b) {
+        if (!(mask >> b & 1)) continue;
+        int i = 0;
+        while (i < P.t && P.tbits[i] != b) ++i;
+        if (i < P.t) loc |= 1u << i;
+        else out |= uint64_t(1) << b;
+      }
+    };
+    TPass cur = P;
+    int nr = 0;
+    auto close = [&](int end) {
+      cur.op_end = end;
+      runs.push_back(cur);
+      run_rows.push_back(nr);
+      run_row0.push_back(rows - nr);
+      cur.op_begin = end;
+      nr = 0;
+    };
+    for (int j = P.op_begin; j < P.op_end; ++j) {
+      const size_t kk = gs.size() - 1 - size_t(origin[j]);
+      TGen& g = gens[j];
+      g = TGen{};
+      g.row = -1;
+      if (gs[kk].param < 0) continue;
+      PauliOp G;
+      double a = 0, b = 0;
       if (!generator(gs[kk], n, G, a, b))
         return fail(QSV_EUNSUP, "gate kind " + std::to_string(gs[kk].kind) + " is not differentiable");
-      // <lam|G|psi> and the un-apply on both states in one pass over them
-      AdjStep step;
-      QSV_TRY(adjoint_step(inv[kk], G, n, step));
-      QSV_TRY(dtype == QSV_C128
-                  ? launch_adjoint_step<double>(psi, lam, n, step, partials + size_t(rows) * kRedBlocks, st)
-                  : launch_adjoint_step<float>(psi, lam, n, step, partials + size_t(rows) * kRedBlocks, st));
+      if (nr == kAdjRowsMax) close(j);
+      uint64_t fout = 0;
+      split(G.flip, g.floc, fout);
+      if (fout) return fail(QSV_EINVAL, "adjoint tiles: generator flip outside the tile");
+      split(G.zmask, g.zloc, g.zout);
+      split(G.pmask, g.ploc, g.pout);
+      split(G.pval, g.pvloc, g.pvout);
+      g.row = nr++;
       // fold the generator's own phase into the weights: z_true = phase * z
       row_param.push_back(gs[kk].param);
       row_wr.push_back(a * G.phase_re + b * G.phase_im);
       row_wi.push_back(-a * G.phase_im + b * G.phase_re);
       ++rows;
-      continue;
     }
-    QSV_TRY(plain_any(dtype, inv[kk], psi, n, st));
-    QSV_TRY(plain_any(dtype, inv[kk], lam, n, st));
+    close(P.op_end);
+  }
+  DevArray<TOp> d_tops;
+  DevArray<TGen> d_gens;
+  QSV_TRY(d_tops.upload(tops));
+  QSV_TRY(d_gens.upload(gens));
+  for (size_t r = 0; r < runs.size(); ++r) {
+    void* part = partials + size_t(run_row0[r]) * kRedBlocks;
+    const TOp* o = d_tops.p + runs[r].op_begin;
+    const TGen* g = d_gens.p + runs[r].op_begin;
+    QSV_TRY(dtype == QSV_C128
+                ? launch_adjoint_tiles<double>(psi, lam, runs[r], o, g, run_rows[r], part, kRedBlocks, st)
+                : launch_adjoint_tiles<float>(psi, lam, runs[r], o, g, run_rows[r], part, kRedBlocks, st));
   }
   QSV_TRY(launch_reduce_rows(partials, rows, red, st));
   std::vector<double2> host(rows);

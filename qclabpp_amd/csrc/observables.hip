@@ -88,70 +88,6 @@ __global__ __launch_bounds__(kRedThreads) void k_pauli_inner(const C* __restrict
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// The fused adjoint step (see AdjStep).  Each thread takes groups of 2^K
-// amplitudes of both states: a group's base index has zeros at the gate's
-// address bits, and consecutive threads take consecutive bases.
-template <typename C, int K>
-__global__ __launch_bounds__(kRedThreads) void k_adjoint_step(C* __restrict__ phi, C* __restrict__ lam,
-                                                              uint64_t ngroups, AdjStep a, double2* partials) {
-  using R = decltype(C::x);
-  constexpr int D = 1 << K;
-  __shared__ double2 sh[kRedThreads];
-  R mr[D * D], mi[D * D];
-#pragma unroll
-  for (int i = 0; i < D * D; ++i) {
-    mr[i] = R(a.m[2 * i]);
-    mi[i] = R(a.m[2 * i + 1]);
-  }
-  double2 acc = make_double2(0, 0);
-  const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
-  for (uint64_t g = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; g < ngroups; g += stride) {
-    uint64_t y0 = g;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {  // insert a zero at each gate bit, lowest first
-      const int b = a.bits[j];
-      const uint64_t lo = y0 & ((uint64_t(1) << b) - 1);
-      y0 = ((y0 ^ lo) << 1) | lo;
-    }
-    C f[D], l[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      f[c] = phi[y0 + a.dep[c]];
-      l[c] = lam[y0 + a.dep[c]];
-    }
-#pragma unroll
-    for (int c = 0; c < D; ++c) {  // sum_c w[c] conj(lam[partner c]) phi[c]
-      // uniform selects instead of a dynamic register index (which would
-      // move l[] to scratch memory)
-      const int pc = a.partner[c];
-      C lp = l[0];
-#pragma unroll
-      for (int e = 1; e < D; ++e)
-        if (pc == e) lp = l[e];
-      const double tr = double(lp.x) * double(f[c].x) + double(lp.y) * double(f[c].y);
-      const double ti = double(lp.x) * double(f[c].y) - double(lp.y) * double(f[c].x);
-      acc.x += a.w[c] * tr;
-      acc.y += a.w[c] * ti;
-    }
-#pragma unroll
-    for (int r = 0; r < D; ++r) {
-      C of, ol;
-      of.x = of.y = ol.x = ol.y = R(0);
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        const R xr = mr[r * D + c], xi = mi[r * D + c];
-        of.x += xr * f[c].x - xi * f[c].y;
-        of.y += xr * f[c].y + xi * f[c].x;
-        ol.x += xr * l[c].x - xi * l[c].y;
-        ol.y += xr * l[c].y + xi * l[c].x;
-      }
-      phi[y0 + a.dep[r]] = of;
-      lam[y0 + a.dep[r]] = ol;
-    }
-  }
-  const double2 s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
 
 // rows x kRedBlocks partials -> rows values, fixed order.
 __global__ __launch_bounds__(kRedThreads) void k_reduce_rows(const double2* __restrict__ partials,
@@ -297,23 +233,6 @@ int launch_pauli_apply(const void* in, void* out, int n, const void* dev_terms,
   return QSV_OK;
 }
 
-template <typename R>
-int launch_adjoint_step(void* psi, void* lam, int n, const AdjStep& a, void* partials, hipStream_t st) {
-  using C = typename CT<R>::type;
-  if (a.K < 1 || a.K > 2 || n < a.K) return fail(QSV_EINVAL, "adjoint step: 1 or 2 gate qubits");
-  const uint64_t ngroups = (uint64_t(1) << n) >> a.K;
-  if (a.K == 1)
-    hipLaunchKernelGGL((k_adjoint_step<C, 1>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, static_cast<C*>(psi),
-                       static_cast<C*>(lam), ngroups, a, static_cast<double2*>(partials));
-  else
-    hipLaunchKernelGGL((k_adjoint_step<C, 2>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, static_cast<C*>(psi),
-                       static_cast<C*>(lam), ngroups, a, static_cast<double2*>(partials));
-  QSV_CHECK_HIP(hipGetLastError());
-  return QSV_OK;
-}
-
-template int launch_adjoint_step<double>(void*, void*, int, const AdjStep&, void*, hipStream_t);
-template int launch_adjoint_step<float>(void*, void*, int, const AdjStep&, void*, hipStream_t);
 template int launch_pauli_inner<double>(const void*, const void*, int, const PauliOp&, void*, hipStream_t);
 template int launch_pauli_inner<float>(const void*, const void*, int, const PauliOp&, void*, hipStream_t);
 template int launch_diag_expect<double>(const void*, int, const std::vector<PauliOp>&, void*, void*, hipStream_t);

@@ -49,25 +49,6 @@ template <typename R>
 int launch_pauli_apply(const void* in, void* out, int n, const void* dev_terms,
                        const void* dev_group_start, int ngroups, int nterms, hipStream_t st);
 
-// One step of the adjoint reverse sweep for a parametrised gate on K = 1 or
-// 2 qubits, in one pass over both states: z = <lam| G |phi> from the values
-// before the step (G the gate's generator, which acts on the gate's own
-// qubits), then phi <- U^-1 phi and lam <- U^-1 lam.  The group of 2^K
-// amplitudes has local index c (bit K-1-j of c is the j-th of the gate's
-// qubits in ascending order, i.e. MSB first); G maps local c to partner[c]
-// with weight w[c] (0 outside its projector); `m` is U^-1 on the group,
-// row-major.  Writes kRedBlocks partial sums of z (FP64).
-struct AdjStep {
-  int K = 1;
-  int bits[2] = {0, 0};     // address bits of the gate qubits, ascending
-  uint64_t dep[4] = {0};    // address offset of local index c
-  int partner[4] = {0};
-  double w[4] = {0};
-  double m[32] = {0};       // 2^K x 2^K complex, row-major, (re, im)
-};
-template <typename R>
-int launch_adjoint_step(void* psi, void* lam, int n, const AdjStep& a, void* partials, hipStream_t st);
-
 // Device image of a Pauli sum for launch_pauli_apply.
 struct DevPauli {
   uint64_t flip, zmask, pmask, pval;

@@ -32,15 +32,18 @@ void set_c(TOp& t, int i, cd z) {
 }  // namespace
 
 int compile_tiled(const std::vector<KOp>& in, int n, int t, bool merge1q, int circuit_id,
-                  std::vector<TPass>& passes, std::vector<TOp>& tops) {
+                  std::vector<TPass>& passes, std::vector<TOp>& tops, std::vector<int>* origin) {
+  if (origin && merge1q) return fail(QSV_EINVAL, "tile compile: origins need unmerged ops");
   if (t < 1) return fail(QSV_EINVAL, "tile bits must be positive");
   t = std::min(t, n);
   if (t > kTileMaxBits) return fail(QSV_EINVAL, "tile too large");
   // 1. optionally merge runs of 1-qubit ops on the same qubit (ops on other
   //    qubits in between commute with them)
   std::vector<KOp> ops;
+  std::vector<int> src;            // index in `in` of each op
   std::vector<int> last1q(n, -1);  // index in `ops` of a mergeable 1-qubit op per qubit
-  for (const KOp& op : in) {
+  for (size_t ii = 0; ii < in.size(); ++ii) {
+    const KOp& op = in[ii];
     if (op.kind == K_NOP) continue;
     const bool one = op.nq == 1 && (op.kind == K_DENSE1 || op.kind == K_DIAG1 || op.kind == K_PHASE1);
     if (merge1q && one && last1q[op.q[0]] >= 0) {
@@ -64,15 +67,18 @@ int compile_tiled(const std::vector<KOp>& in, int n, int t, bool merge1q, int ci
     }
     for (int q : all_qubits(op)) last1q[q] = -1;
     ops.push_back(op);
+    src.push_back(int(ii));
     if (one) last1q[op.q[0]] = int(ops.size()) - 1;
   }
   // 2. passes
   const int first_pass = int(passes.size());
   std::vector<KOp> rem = ops;
+  std::vector<int> rem_src = src;
   const int c_low = std::min(5, t);
   bool first = true;
   if (rem.empty()) {  // the identity circuit still needs its |0..0> written
     rem.push_back(KOp());
+    rem_src.push_back(-1);
   }
   while (!rem.empty()) {
     std::vector<int> tq;  // tile qubits
@@ -129,7 +135,9 @@ int compile_tiled(const std::vector<KOp>& in, int n, int t, bool merge1q, int ci
     P.ntiles = uint64_t(1) << (n - t);
     std::set<int> blocked;
     std::vector<KOp> deferred;
-    for (const KOp& op : rem) {
+    std::vector<int> deferred_src;
+    for (size_t ri = 0; ri < rem.size(); ++ri) {
+      const KOp& op = rem[ri];
       bool ok = op.kind != K_NOP;
       for (int q : local_qubits(op)) ok = ok && tset.count(q);
       for (int q : all_qubits(op)) ok = ok && !blocked.count(q);
@@ -137,8 +145,10 @@ int compile_tiled(const std::vector<KOp>& in, int n, int t, bool merge1q, int ci
       if (!ok) {
         for (int q : all_qubits(op)) blocked.insert(q);
         deferred.push_back(op);
+        deferred_src.push_back(rem_src[ri]);
         continue;
       }
+      if (origin) origin->push_back(rem_src[ri]);
       TOp to{};
       switch (op.kind) {
         case K_DENSE1:
@@ -186,6 +196,7 @@ int compile_tiled(const std::vector<KOp>& in, int n, int t, bool merge1q, int ci
         rem[0].kind != K_NOP)
       return fail(QSV_EINVAL, "tile compile made no progress");
     rem.swap(deferred);
+    rem_src.swap(deferred_src);
   }
   passes.back().last = 1;
   (void)first_pass;

@@ -171,6 +171,105 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_level(
   }
 }
 
+// Deterministic sum over a wavefront (fixed butterfly order); every lane
+// ends with the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One pass of the adjoint reverse sweep over tiles of both states: for each
+// op, the inner product <lam|G|phi> of its generator over the tile (before
+// the op), then the op on both tiles.  Per-wave accumulators in LDS, summed
+// per block in wave order at the end: partials[row * gridDim.x + block].
+template <typename C>
+__global__ __launch_bounds__(kTileThreads) void k_adjoint_tiles(C* __restrict__ phi, C* __restrict__ lam,
+                                                                const TPass P, const TOp* __restrict__ ops,
+                                                                const TGen* __restrict__ gens, int nrows,
+                                                                double2* __restrict__ partials) {
+  extern __shared__ unsigned char smem[];
+  const int t = P.t;
+  const uint32_t size = 1u << t;
+  C* Lf = reinterpret_cast<C*>(smem);
+  C* Ll = Lf + size;
+  double2* acc = reinterpret_cast<double2*>(Ll + size);  // [nrows][4 waves]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < nrows * 4; i += kTileThreads) acc[i] = make_double2(0, 0);
+  // element e = threadIdx.x + 256*k: offset = pdep(low 8 bits) | pdep(k << 8)
+  uint64_t lane_off = 0;
+  for (int i = 0; i < t && i < 8; ++i)
+    if (threadIdx.x >> i & 1u) lane_off |= uint64_t(1) << P.tbits[i];
+  __syncthreads();
+  const int nops = P.op_end - P.op_begin;
+  for (uint64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x) {
+    uint64_t base = 0;
+    {
+      int sh = 0;
+      for (int r = 0; r < P.nruns; ++r) {
+        base |= ((tile >> sh) & ((uint64_t(1) << P.rlen[r]) - 1)) << P.rdst[r];
+        sh += P.rlen[r];
+      }
+    }
+    for (uint32_t e = threadIdx.x, k = 0; e < size; e += kTileThreads, ++k) {
+      uint64_t off = lane_off;
+      for (int i = 8; i < t; ++i)
+        if (k >> (i - 8) & 1u) off |= uint64_t(1) << P.tbits[i];
+      Lf[e] = phi[base + off];
+      Ll[e] = lam[base + off];
+    }
+    __syncthreads();
+    for (int o = 0; o < nops; ++o) {
+      const TGen g = gens[o];
+      if (g.row >= 0) {  // block-uniform
+        double sr = 0, si = 0;
+        if ((base & g.pout) == g.pvout) {
+          const int zs = __popcll(base & g.zout) & 1;
+          for (uint32_t e = threadIdx.x; e < size; e += kTileThreads) {
+            if ((e & g.ploc) != g.pvloc) continue;
+            const C f = Lf[e], l = Ll[e ^ g.floc];
+            double tr = double(l.x) * double(f.x) + double(l.y) * double(f.y);
+            double ti = double(l.x) * double(f.y) - double(l.y) * double(f.x);
+            if ((__popc(e & g.zloc) & 1) ^ zs) {
+              tr = -tr;
+              ti = -ti;
+            }
+            sr += tr;
+            si += ti;
+          }
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) {
+          acc[g.row * 4 + wave].x += sr;
+          acc[g.row * 4 + wave].y += si;
+        }
+        __syncthreads();  // the op below rewrites what other threads just read
+      }
+      apply_top<C>(ops[o], Lf, size, base);
+      apply_top<C>(ops[o], Ll, size, base);
+      __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x, k = 0; e < size; e += kTileThreads, ++k) {
+      uint64_t off = lane_off;
+      for (int i = 8; i < t; ++i)
+        if (k >> (i - 8) & 1u) off |= uint64_t(1) << P.tbits[i];
+      phi[base + off] = Lf[e];
+      lam[base + off] = Ll[e];
+    }
+    __syncthreads();
+  }
+  for (int r = threadIdx.x; r < nrows; r += kTileThreads) {
+    double2 s = acc[r * 4];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      s.x += acc[r * 4 + w].x;
+      s.y += acc[r * 4 + w].y;
+    }
+    partials[size_t(r) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
 // Reset the ticket and completion counters of one run and stamp its start;
 // a kernel rather than a memset so the run can be captured in a graph.
 __global__ void k_tile_prologue(unsigned long long* tickets, int ntickets, unsigned int* done,
@@ -206,6 +305,25 @@ int launch_tile_prologue(unsigned long long* tickets, int ntickets, unsigned int
   return QSV_OK;
 }
 
+template <typename R>
+int launch_adjoint_tiles(void* phi, void* lam, const TPass& pass, const TOp* ops, const TGen* gens,
+                         int nrows, void* partials, int blocks, hipStream_t st) {
+  using C = typename CT<R>::type;
+  if (nrows < 0 || nrows > kAdjRowsMax) return fail(QSV_EINVAL, "adjoint tiles: too many rows in one pass");
+  if (pass.t < 1 || pass.t > kTileMaxBits) return fail(QSV_EINVAL, "adjoint tiles: bad tile size");
+  const size_t lds = 2 * (size_t(1) << pass.t) * sizeof(C) + size_t(kAdjRowsMax) * 4 * sizeof(double2);
+  QSV_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adjoint_tiles<C>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipLaunchKernelGGL((k_adjoint_tiles<C>), dim3(blocks), dim3(kTileThreads), lds, st, static_cast<C*>(phi),
+                     static_cast<C*>(lam), pass, ops, gens, nrows, static_cast<double2*>(partials));
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+template int launch_adjoint_tiles<double>(void*, void*, const TPass&, const TOp*, const TGen*, int, void*, int,
+                                          hipStream_t);
+template int launch_adjoint_tiles<float>(void*, void*, const TPass&, const TOp*, const TGen*, int, void*, int,
+                                         hipStream_t);
 template int launch_tile_level<double>(const TileLevelArgs&, hipStream_t);
 template int launch_tile_level<float>(const TileLevelArgs&, hipStream_t);
 

@@ -59,7 +59,27 @@ struct TItem {
 
 // Host compile of one circuit into passes over tiles of `t` bits.
 // `merge1q` multiplies runs of 1-qubit gates on the same qubit into one.
+// `origin` (optional, with merge1q false): index in `ops` of each TOp.
 int compile_tiled(const std::vector<KOp>& ops, int n, int t, bool merge1q, int circuit_id,
-                  std::vector<TPass>& passes, std::vector<TOp>& tops);
+                  std::vector<TPass>& passes, std::vector<TOp>& tops, std::vector<int>* origin = nullptr);
+
+// The adjoint reverse sweep in LDS tile passes: both states' tiles are in
+// LDS, and each op is preceded, when it carries a parameter, by its
+// generator's inner product <lam|G|phi> over the tile.  G's masks are split
+// into tile-local bits (`*loc`) and address bits outside the tile (`*out`,
+// fixed per tile).  `row` is the op's result row within its pass, -1 if none.
+struct TGen {
+  int32_t row;
+  uint32_t floc, zloc, ploc, pvloc;
+  int32_t pad;
+  uint64_t zout, pout, pvout;
+};
+constexpr int kAdjRowsMax = 128;  // result rows per pass (LDS accumulators)
+
+// One pass: grid of `blocks` blocks (each takes tiles b, b + blocks, ...),
+// partial sums of row r of this pass at partials[r * blocks + block].
+template <typename R>
+int launch_adjoint_tiles(void* phi, void* lam, const TPass& pass, const TOp* ops, const TGen* gens,
+                         int nrows, void* partials, int blocks, hipStream_t st);
 
 }  // namespace qsv

@@ -86,10 +86,14 @@ def test_adjoint_vs_param_shift_tfim(fuse_k):
     assert abs(back[0] - 1) < 1e-12 and np.abs(back[1:]).max() < 1e-12
 
 
-def test_adjoint_all_differentiable_kinds_vs_fd():
+@pytest.mark.parametrize("n", [6, 15])
+def test_adjoint_all_differentiable_kinds_vs_fd(n):
+    """Every differentiable kind against central differences.  At n=15 the
+    reverse sweep's tiles (2^11 amplitudes) leave qubits outside the tile, so
+    generators and controls on address bits outside it are covered."""
     import torch
     from qclabpp_amd.autograd import finite_difference
-    n = 6
+    print(f"adjoint kinds n={n}", flush=True)
     rs = np.random.default_rng(4)
     circ = [G.Hadamard(q) for q in range(n)]
     p = 0
@@ -101,11 +105,13 @@ def test_adjoint_all_differentiable_kinds_vs_fd():
         for q in range(n - 1):
             circ.append(ctor(q, q + 1, rs.uniform(-3, 3), param=p))
             p += 1
+        circ.append(ctor(0, n - 1, rs.uniform(-3, 3), param=p))
+        p += 1
     for ctor in (G.CRotationX, G.CRotationY, G.CRotationZ, G.CPhase):
         for cs in (1, 0):
-            circ.append(ctor(2, 4, rs.uniform(-3, 3), controlState=cs, param=p))
-            circ.append(ctor(5, 1, rs.uniform(-3, 3), controlState=cs, param=p + 1))
-            p += 2
+            for c, t in ((2, 4), (5, 1), (0, n - 1), (n - 1, 3)):
+                circ.append(ctor(c, t, rs.uniform(-3, 3), controlState=cs, param=p))
+                p += 1
     circ.append(G.RotationY(0, 0.7, param=0))  # a shared parameter
     theta = np.array([next(g.theta for g in circ if g.param == i) for i in range(p)])
     H = mixed_H(n)
@@ -113,6 +119,35 @@ def test_adjoint_all_differentiable_kinds_vs_fd():
     lam = torch.empty_like(psi)
     e, g = O.adjoint_gradient(psi, lam, circ, H, theta=theta, fuse_k=0)
     fd = finite_difference(circ, H, theta, n, h=1e-6, fuse_k=0)
+    assert np.abs(g - fd).max() < 1e-8
+
+
+def test_adjoint_pass_with_many_parametrised_gates_vs_fd():
+    """384 parametrised gates on 11 qubits of a 13-qubit state fit one tile
+    pass of the reverse sweep, which then runs in several launches of at
+    most 128 gradient rows each."""
+    import torch
+    from qclabpp_amd.autograd import finite_difference
+    n = 13
+    rs = np.random.default_rng(12)
+    circ = [G.Hadamard(q) for q in range(n)]
+    p = 0
+    for _ in range(12):
+        for q in range(2, n):
+            circ.append(G.RotationY(q, rs.uniform(-3, 3), param=p))
+            circ.append(G.RotationZ(q, rs.uniform(-3, 3), param=p + 1))
+            p += 2
+        for q in range(2, n - 1):
+            circ.append(G.RotationZZ(q, q + 1, rs.uniform(-3, 3), param=p))
+            p += 1
+    assert p == 384
+    theta = np.array([next(g.theta for g in circ if g.param == i) for i in range(p)])
+    H = mixed_H(n)
+    psi = E.new_state(n)
+    lam = torch.empty_like(psi)
+    e, g = O.adjoint_gradient(psi, lam, circ, H, theta=theta, fuse_k=0)
+    fd = finite_difference(circ, H, theta, n, h=1e-6, fuse_k=0)
+    assert abs(e - oracle.energy(n, circ, H, theta)) < 1e-11
     assert np.abs(g - fd).max() < 1e-8
 
 
@@ -131,7 +166,7 @@ def test_autograd_function():
 
 
 def test_adjoint_complex64_vs_complex128():
-    """The fused reverse-sweep step in FP32 arithmetic with FP64 inner
+    """The reverse sweep's tile passes in FP32 arithmetic with FP64 inner
     products, against the complex128 gradient."""
     import torch
     n = 10
