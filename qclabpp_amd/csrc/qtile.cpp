@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdlib>
 #include <set>
 #include <string>
@@ -567,10 +568,16 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
               scale([&](int i) { return on(i) ? (bit(i, t) ? m[2] : m[1]) : cd(1.0); });
               continue;
             }
-            flush_diag();
-            QOp o{};
             const int d = fr.phys[1 << t] ^ fr.phys[0];
             const int K = d & -d;
+            // pending diagonal factors that are equal on both members of
+            // every pair the op updates commute with it: they stay pending
+            // and merge with later ones
+            bool commutes = true;
+            for (int p = 0; p < NR; ++p)
+              if (!(p & K) && on(fr.lg[p]) && diag[p] != diag[p ^ d]) commutes = false;
+            if (!commutes) flush_diag();
+            QOp o{};
             o.d = d;
             int nbase = 0;
             for (int p = 0; p < NR; ++p) {
@@ -619,7 +626,10 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
               xa = t0; fa = 0;
               xb = t1; fb = 0;
             }
-            flush_diag();
+            bool commutes = true;  // pending factors constant on every quad
+            for (int p = 0; p < NR; ++p)
+              if (diag[p] != diag[p ^ (1 << xa)] || diag[p] != diag[p ^ (1 << xb)]) commutes = false;
+            if (!commutes) flush_diag();
             QOp o{};
             o.kind = Q_DENSE2;
             o.a = xa;
@@ -652,6 +662,75 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
 }
 
 }  // namespace
+
+void qtile_scale_rotations(QProgram& qp) {
+  constexpr int NR = 1 << kQRegBits;
+  std::vector<QOp> ops;
+  ops.reserve(qp.ops.size() + qp.passes.size());
+  for (const QPass& P : qp.passes) {
+    cd lam = 1.0;  // product of the factors taken out so far in this pass
+    for (int ph = P.phase_begin; ph < P.phase_end; ++ph) {
+      QPhase& Q = qp.phases[ph];
+      const int begin = int(ops.size());
+      for (int o = Q.op_begin; o < Q.op_end; ++o) {
+        QOp x = qp.ops[o];
+        // rotation: c = re(m0), s = re(m2); RX-like: p = re(m0) = re(m3), q = im(m1) = im(m2);
+        // unitary ones only (c^2 + s^2 = 1), which bound the growth
+        const double c = x.m[0], sn = x.kind == Q_ROTP ? x.m[4] : x.m[3];
+        const bool unit = std::abs(c * c + sn * sn - 1.0) < 1e-12;
+        if (unit && (x.kind == Q_ROTP || (x.kind == Q_XLIKEP && x.m[0] == x.m[6] && x.m[3] == x.m[5]))) {
+          const bool first = std::abs(c) >= std::abs(sn);
+          const double f = first ? c : sn;
+          x.kind = x.kind == Q_ROTP ? Q_ROTS : Q_XLS;
+          for (int i = 0; i < 16; ++i) set_c(x, i, 0.0);
+          set_c(x, 0, first ? sn / c : c / sn);
+          set_c(x, 1, first ? 1.0 : 0.0);
+          lam *= f;
+        } else if (x.kind == Q_DIAGR) {
+          // take out the most common entry (unit entries of registers outside b included)
+          cd e[NR];
+          for (int p = 0; p < NR; ++p) e[p] = x.b >> p & 1 ? cd(x.m[2 * p], x.m[2 * p + 1]) : cd(1.0);
+          int best = 0, nbest = 0, nunit = 0;
+          for (int p = 0; p < NR; ++p) {
+            int cnt = 0;
+            for (int r = 0; r < NR; ++r) cnt += e[r] == e[p];
+            if (cnt > nbest) best = p, nbest = cnt;
+            nunit += e[p] == cd(1.0);
+          }
+          if (nbest > nunit) {
+            const cd f = e[best];
+            x.b = 0;
+            for (int p = 0; p < NR; ++p) {
+              const cd z = e[p] == f ? cd(1.0) : e[p] / f;
+              set_c(x, p, z);
+              if (z != cd(1.0)) x.b |= 1 << p;
+            }
+            lam *= f;
+            if (!x.b) continue;
+          }
+        }
+        ops.push_back(x);
+      }
+      if (ph == P.phase_end - 1 && lam != cd(1.0)) {
+        if (int(ops.size()) > begin && ops.back().kind == Q_DIAGR) {
+          QOp& d = ops.back();
+          for (int p = 0; p < NR; ++p)
+            set_c(d, p, (d.b >> p & 1 ? cd(d.m[2 * p], d.m[2 * p + 1]) : cd(1.0)) * lam);
+          d.b = (1 << NR) - 1;
+        } else {
+          QOp d{};
+          d.kind = Q_DIAGR;
+          for (int p = 0; p < NR; ++p) set_c(d, p, lam);
+          d.b = (1 << NR) - 1;
+          ops.push_back(d);
+        }
+      }
+      Q.op_begin = begin;
+      Q.op_end = int(ops.size());
+    }
+  }
+  qp.ops.swap(ops);
+}
 
 int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out, int tile_bits,
                   bool relabel) {

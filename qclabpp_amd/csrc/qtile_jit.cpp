@@ -74,6 +74,37 @@ __device__ __forceinline__ void u_xl(C& a, C& b, R p, R q, R r, R s) {
   a = mk(p * x.x - q * y.y, p * x.y + q * y.x);
   b = mk(s * y.x - r * x.y, s * y.y + r * x.x);
 }
+// scaled forms (the factor taken out is applied at the end of the pass):
+// [[1, -k], [k, 1]]
+__device__ __forceinline__ void u_ta(C& a, C& b, R k) {
+  const C x = a, y = b;
+  a = mk(x.x - k * y.x, x.y - k * y.y);
+  b = mk(y.x + k * x.x, y.y + k * x.y);
+}
+// [[k, -1], [1, k]]
+__device__ __forceinline__ void u_tb(C& a, C& b, R k) {
+  const C x = a, y = b;
+  a = mk(k * x.x - y.x, k * x.y - y.y);
+  b = mk(k * y.x + x.x, k * y.y + x.y);
+}
+// [[k, 1], [-1, k]]
+__device__ __forceinline__ void u_tbs(C& a, C& b, R k) {
+  const C x = a, y = b;
+  a = mk(k * x.x + y.x, k * x.y + y.y);
+  b = mk(k * y.x - x.x, k * y.y - x.y);
+}
+// [[1, ik], [ik, 1]]
+__device__ __forceinline__ void u_xa(C& a, C& b, R k) {
+  const C x = a, y = b;
+  a = mk(x.x - k * y.y, x.y + k * y.x);
+  b = mk(y.x - k * x.y, y.y + k * x.x);
+}
+// [[k, i], [i, k]]
+__device__ __forceinline__ void u_xb(C& a, C& b, R k) {
+  const C x = a, y = b;
+  a = mk(k * x.x - y.y, k * x.y + y.x);
+  b = mk(k * y.x - x.y, k * y.y + x.x);
+}
 // 4x4 on (a0, a1, a2, a3), coefficients row-major from cf[o]
 __device__ __forceinline__ void u_d2(C& a0, C& a1, C& a2, C& a3, const QC R* cf, int o) {
   const C x[4] = {a0, a1, a2, a3};
@@ -126,6 +157,25 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128) {
         s << "      u_xl(" << v(p) << ", " << v(q) << (sw ? ", xs, xr, xq, xp);\n" : ", xp, xq, xr, xs);\n");
       });
       break;
+    case Q_ROTS:
+    case Q_XLS: {
+      // the form is data (it depends on the angle), so one compiled pass
+      // serves every angle: a branch on a uniform value
+      const bool rot = o.kind == Q_ROTS;
+      s << "      const R k = cf[" << at(0) << "];\n"
+        << "      if (cf[" << at(2) << "] != R(0)) {\n";
+      pairs([&](int p, int q, bool sw, bool) {
+        if (rot) s << "        u_ta(" << v(p) << ", " << v(q) << (sw ? ", -k);\n" : ", k);\n");
+        else s << "        u_xa(" << v(p) << ", " << v(q) << ", k);\n";
+      });
+      s << "      } else {\n";
+      pairs([&](int p, int q, bool sw, bool) {
+        if (rot) s << "        " << (sw ? "u_tbs(" : "u_tb(") << v(p) << ", " << v(q) << ", k);\n";
+        else s << "        u_xb(" << v(p) << ", " << v(q) << ", k);\n";
+      });
+      s << "      }\n";
+      break;
+    }
     case Q_PAIR:
     case Q_CPAIR:
       s << "      const C m0 = cf2(cf, " << at(0) << "), m1 = cf2(cf, " << at(2) << "), m2 = cf2(cf, "
