@@ -279,6 +279,11 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
         g.theta = theta[g.param];
         g.has_cs = 0;
       }
+      // reject a gate without a generator before psi is touched
+      PauliOp G;
+      double a = 0, b = 0;
+      if (!generator(g, n, G, a, b))
+        return fail(QSV_EUNSUP, "gate kind " + std::to_string(g.kind) + " is not differentiable");
     }
   }
   // forward: phi = U psi0
@@ -358,6 +363,31 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
       split(G.zmask, g.zloc, g.zout);
       split(G.pmask, g.ploc, g.pout);
       split(G.pval, g.pvloc, g.pvout);
+      // the inner product fits inside the op's own loop when the flip stays
+      // in the thread's pair or quad and the pairs a controlled op skips lie
+      // outside the projector
+      const TOp& t = tops[j];
+      const uint32_t tgt = t.a >= 0 ? 1u << t.a : 0u;
+      switch (t.kind) {
+        case T_DENSE1:
+          g.fused = g.floc == 0 || g.floc == tgt;
+          break;
+        case T_CDENSE1: {
+          const bool proj = t.b >= 0 ? ((g.ploc >> t.b & 1u) && int(g.pvloc >> t.b & 1u) == t.cs)
+                                     : ((g.pout >> ~t.b & 1u) && int(g.pvout >> ~t.b & 1u) == t.cs);
+          g.fused = proj && (g.floc == 0 || g.floc == tgt);
+          break;
+        }
+        case T_DIAG1:
+        case T_DIAG2:
+          g.fused = g.floc == 0;
+          break;
+        case T_DENSE2:
+          g.fused = t.b >= 0 && (g.floc & ~(tgt | (1u << t.b))) == 0;
+          break;
+        default:
+          g.fused = 0;
+      }
       g.row = nr++;
       // fold the generator's own phase into the weights: z_true = phase * z
       row_param.push_back(gs[kk].param);

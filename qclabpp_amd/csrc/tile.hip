@@ -171,6 +171,109 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_level(
   }
 }
 
+// The generator's inner product, accumulated from the values before an op
+// by the thread that updates them (see TGen::fused).
+struct GenAcc {
+  bool on;        // the tile lies inside the generator's projector
+  int zs;         // parity of the Z mask outside the tile
+  uint32_t floc, zloc, ploc, pvloc;
+  double sr, si;
+  template <typename C>
+  __device__ __forceinline__ void add(uint32_t e, C f, C lp) {  // lp = lam at e ^ floc
+    if (!on || (e & ploc) != pvloc) return;
+    double tr = double(lp.x) * double(f.x) + double(lp.y) * double(f.y);
+    double ti = double(lp.x) * double(f.y) - double(lp.y) * double(f.x);
+    if ((__popc(e & zloc) & 1) ^ zs) {
+      tr = -tr;
+      ti = -ti;
+    }
+    sr += tr;
+    si += ti;
+  }
+};
+
+// apply_top on both tiles in one loop, adding the generator's inner product
+// over the elements each thread updates when ga.on.
+template <typename C>
+__device__ void apply2_top(const TOp& o, C* F, C* L, uint32_t size, uint64_t base, GenAcc& ga) {
+  const int tid = threadIdx.x;
+  switch (o.kind) {
+    case T_DENSE1:
+    case T_CDENSE1: {
+      const C m0 = coef<C>(o, 0), m1 = coef<C>(o, 1), m2 = coef<C>(o, 2), m3 = coef<C>(o, 3);
+      const uint32_t s = 1u << o.a;
+      const bool ctl = o.kind == T_CDENSE1;
+      const bool fl = ga.floc != 0;  // the generator flips the target (floc == s)
+      for (uint32_t p = tid; p < size / 2; p += kTileThreads) {
+        const uint32_t i0 = uint32_t(ins0(p, o.a)), i1 = i0 | s;
+        if (ctl && obit(o.b, i0, base) != o.cs) continue;
+        const C f0 = F[i0], f1 = F[i1], l0 = L[i0], l1 = L[i1];
+        ga.add(i0, f0, fl ? l1 : l0);
+        ga.add(i1, f1, fl ? l0 : l1);
+        F[i0] = cmad(m1, f1, cmul(m0, f0));
+        F[i1] = cmad(m3, f1, cmul(m2, f0));
+        L[i0] = cmad(m1, l1, cmul(m0, l0));
+        L[i1] = cmad(m3, l1, cmul(m2, l0));
+      }
+      break;
+    }
+    case T_DIAG1:
+    case T_DIAG2: {
+      const C d0 = coef<C>(o, 0), d1 = coef<C>(o, 1);
+      const C d2 = o.kind == T_DIAG2 ? coef<C>(o, 2) : d0, d3 = o.kind == T_DIAG2 ? coef<C>(o, 3) : d1;
+      for (uint32_t e = tid; e < size; e += kTileThreads) {
+        const int ba = obit(o.a, e, base), bb = o.kind == T_DIAG2 ? obit(o.b, e, base) : 0;
+        const C d = o.kind == T_DIAG2 ? (ba ? (bb ? d3 : d2) : (bb ? d1 : d0)) : (ba ? d1 : d0);
+        const C f = F[e], l = L[e];
+        ga.add(e, f, l);
+        F[e] = cmul(d, f);
+        L[e] = cmul(d, l);
+      }
+      break;
+    }
+    case T_DENSE2: {
+      const int lo = o.a < o.b ? o.a : o.b, hi = o.a < o.b ? o.b : o.a;
+      // quad member r: bit 1 of r is bit a, bit 0 is bit b; the partner of r
+      // under the generator's flip is r ^ fr
+      const int fr = int((ga.floc >> o.a) & 1u) << 1 | int((ga.floc >> o.b) & 1u);
+      for (uint32_t p = tid; p < size / 4; p += kTileThreads) {
+        const uint32_t i = uint32_t(ins0(ins0(p, lo), hi));
+        const uint32_t idx[4] = {i, i | (1u << o.b), i | (1u << o.a), i | (1u << o.a) | (1u << o.b)};
+        C x[4], y[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          x[r] = F[idx[r]];
+          y[r] = L[idx[r]];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          C lp = y[0];
+#pragma unroll
+          for (int q = 1; q < 4; ++q)
+            if ((r ^ fr) == q) lp = y[q];
+          ga.add(idx[r], x[r], lp);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          C af = cmul(coef<C>(o, 4 * r), x[0]), al = cmul(coef<C>(o, 4 * r), y[0]);
+#pragma unroll
+          for (int c = 1; c < 4; ++c) {
+            af = cmad(coef<C>(o, 4 * r + c), x[c], af);
+            al = cmad(coef<C>(o, 4 * r + c), y[c], al);
+          }
+          F[idx[r]] = af;
+          L[idx[r]] = al;
+        }
+      }
+      break;
+    }
+    default:  // T_SWAP and the rest: never fused with a generator
+      apply_top<C>(o, F, size, base);
+      apply_top<C>(o, L, size, base);
+      break;
+  }
+}
+
 // Deterministic sum over a wavefront (fixed butterfly order); every lane
 // ends with the total.
 __device__ __forceinline__ double wave_sum(double v) {
@@ -221,6 +324,17 @@ __global__ __launch_bounds__(kTileThreads) void k_adjoint_tiles(C* __restrict__ 
     __syncthreads();
     for (int o = 0; o < nops; ++o) {
       const TGen g = gens[o];
+      if (g.row >= 0 && g.fused) {  // block-uniform: inner product inside the update
+        GenAcc ga{(base & g.pout) == g.pvout, int(__popcll(base & g.zout) & 1), g.floc, g.zloc, g.ploc, g.pvloc, 0.0, 0.0};
+        apply2_top<C>(ops[o], Lf, Ll, size, base, ga);
+        const double sr = wave_sum(ga.sr), si = wave_sum(ga.si);
+        if (lane == 0) {
+          acc[g.row * 4 + wave].x += sr;
+          acc[g.row * 4 + wave].y += si;
+        }
+        __syncthreads();
+        continue;
+      }
       if (g.row >= 0) {  // block-uniform
         double sr = 0, si = 0;
         if ((base & g.pout) == g.pvout) {
@@ -246,8 +360,8 @@ __global__ __launch_bounds__(kTileThreads) void k_adjoint_tiles(C* __restrict__ 
         }
         __syncthreads();  // the op below rewrites what other threads just read
       }
-      apply_top<C>(ops[o], Lf, size, base);
-      apply_top<C>(ops[o], Ll, size, base);
+      GenAcc none{false, 0, 0u, 0u, 0u, 0u, 0.0, 0.0};
+      apply2_top<C>(ops[o], Lf, Ll, size, base, none);
       __syncthreads();
     }
     for (uint32_t e = threadIdx.x, k = 0; e < size; e += kTileThreads, ++k) {

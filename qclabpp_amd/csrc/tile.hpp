@@ -68,10 +68,15 @@ int compile_tiled(const std::vector<KOp>& ops, int n, int t, bool merge1q, int c
 // generator's inner product <lam|G|phi> over the tile.  G's masks are split
 // into tile-local bits (`*loc`) and address bits outside the tile (`*out`,
 // fixed per tile).  `row` is the op's result row within its pass, -1 if none.
+// `fused`: the thread that updates an element also holds its partner under
+// the generator's flip (inside the op's pair or quad), and the elements the
+// op skips lie outside the generator's projector, so the inner product is
+// accumulated inside the op's own loop from the values before the update;
+// otherwise it takes a sweep of its own before the op.
 struct TGen {
   int32_t row;
   uint32_t floc, zloc, ploc, pvloc;
-  int32_t pad;
+  int32_t fused;
   uint64_t zout, pout, pvout;
 };
 constexpr int kAdjRowsMax = 128;  // result rows per pass (LDS accumulators)

@@ -151,6 +151,21 @@ def test_adjoint_pass_with_many_parametrised_gates_vs_fd():
     assert np.abs(g - fd).max() < 1e-8
 
 
+def test_adjoint_rejects_non_differentiable_gate_before_touching_psi():
+    import torch
+    from qclabpp_amd._lib import QsvError
+    n = 6
+    h = G.Hadamard(2)
+    h.param = 0
+    circ = [G.RotationY(0, 0.4, param=1), h]
+    psi0 = random_state(n, 21)
+    psi = dev(psi0)
+    lam = torch.empty_like(psi)
+    with pytest.raises(QsvError, match="not differentiable"):
+        O.adjoint_gradient(psi, lam, circ, O.tfim(n), fuse_k=0)
+    assert np.array_equal(psi.cpu().numpy(), psi0)
+
+
 def test_autograd_function():
     import torch
     from qclabpp_amd.autograd import energy
