@@ -180,11 +180,13 @@ def test_autograd_function():
     assert np.abs(theta.grad.numpy() - ps).max() < 1e-12
 
 
-def test_adjoint_complex64_vs_complex128():
+@pytest.mark.parametrize("n", [10, 13])
+def test_adjoint_complex64_vs_complex128(n):
     """The reverse sweep's tile passes in FP32 arithmetic with FP64 inner
-    products, against the complex128 gradient."""
+    products, against the complex128 gradient (n=13: full-size tiles of both
+    precisions, 2^11 and 2^12 amplitudes)."""
     import torch
-    n = 10
+    print(f"adjoint c64 vs c128 n={n}", flush=True)
     circ = W.tfim_ansatz(n, 3, seed=9)
     H = O.tfim(n)
     out = {}
