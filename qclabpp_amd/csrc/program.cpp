@@ -83,7 +83,7 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
     // leave one block per CU, measured 3% slower despite 15 passes for 17)
     const int tb = jit ? jit_tile_bits(dtype, n) : kQTileBits;
     QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt, tb, jit));
-    if (jit) qtile_scale_rotations(qt);
+    qtile_scale_rotations(qt);
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     qt_pass_off = 0;
     qt_phase_off = up(qt.passes.size() * sizeof(QPass));

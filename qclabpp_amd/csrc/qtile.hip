@@ -146,6 +146,61 @@ __device__ __forceinline__ void q_xlikep(C (&v)[NR], const QCONST QOp& o) {
     }
 }
 
+// scaled rotation (Q_ROTS): [[1, -k], [k, 1]] or [[k, -1], [1, k]] as re(m1)
+// is 1 or 0; swapping rows and columns negates the off-diagonal entries
+template <int D, bool SEL, typename C>
+__device__ __forceinline__ void q_rots(C (&v)[NR], const QCONST QOp& o) {
+  using R = decltype(C::x);
+  constexpr int K = D & -D;
+  const R k = qre<C>(o, 0);
+  const uint32_t roles = o.roles;
+  if (qre<C>(o, 2) != R(0)) {
+#pragma unroll
+    for (int p = 0; p < NR; ++p)
+      if (!(p & K)) {
+        const R t = SEL && (roles >> p & 1) ? -k : k;
+        const C x = v[p], y = v[p ^ D];
+        v[p] = mk<C>(x.x - t * y.x, x.y - t * y.y);
+        v[p ^ D] = mk<C>(y.x + t * x.x, y.y + t * x.y);
+      }
+  } else {
+#pragma unroll
+    for (int p = 0; p < NR; ++p)
+      if (!(p & K)) {
+        const R g = SEL && (roles >> p & 1) ? R(-1) : R(1);
+        const C x = v[p], y = v[p ^ D];
+        v[p] = mk<C>(k * x.x - g * y.x, k * x.y - g * y.y);
+        v[p ^ D] = mk<C>(k * y.x + g * x.x, k * y.y + g * x.y);
+      }
+  }
+}
+
+// scaled RX-like (Q_XLS): [[1, ik], [ik, 1]] or [[k, i], [i, k]] as re(m1)
+// is 1 or 0 (symmetric, so roles do not matter)
+template <int D, bool SEL, typename C>
+__device__ __forceinline__ void q_xls(C (&v)[NR], const QCONST QOp& o) {
+  using R = decltype(C::x);
+  constexpr int K = D & -D;
+  const R k = qre<C>(o, 0);
+  if (qre<C>(o, 2) != R(0)) {
+#pragma unroll
+    for (int p = 0; p < NR; ++p)
+      if (!(p & K)) {
+        const C x = v[p], y = v[p ^ D];
+        v[p] = mk<C>(x.x - k * y.y, x.y + k * y.x);
+        v[p ^ D] = mk<C>(y.x - k * x.y, y.y + k * x.x);
+      }
+  } else {
+#pragma unroll
+    for (int p = 0; p < NR; ++p)
+      if (!(p & K)) {
+        const C x = v[p], y = v[p ^ D];
+        v[p] = mk<C>(k * x.x - y.y, k * x.y + y.x);
+        v[p ^ D] = mk<C>(k * y.x - x.y, k * y.y + x.x);
+      }
+  }
+}
+
 // 4x4 on register bits (A, B), row index 2*bit(A) + bit(B)
 template <int A, int B, typename C>
 __device__ __forceinline__ void q_dense2(C (&v)[NR], const QCONST QOp& o) {
@@ -205,6 +260,12 @@ __device__ __forceinline__ void apply_op(const QCONST QOp& o, C (&v)[NR]) {
       break;
     case Q_XLIKEP:
       QBYDIST(q_xlikep);
+      break;
+    case Q_ROTS:
+      QBYDIST(q_rots);
+      break;
+    case Q_XLS:
+      QBYDIST_SEL(q_xls, false);
       break;
     case Q_REALP:
       QBYDIST(q_realp);

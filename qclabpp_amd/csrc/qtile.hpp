@@ -40,8 +40,8 @@ enum QKind : int32_t {
   Q_REALP = 7,    // real 2x2: re(m[0..3])
   Q_ROTP = 8,     // real rotation [[c, -s], [s, c]]: c = re(m0), s = re(m2)
   Q_XLIKEP = 9,   // [[p, iq], [ir, s]], p q r s real: re(m0) im(m1) im(m2) re(m3)
-  // Scaled forms (compiled passes only, see qtile_scale_rotations): the op
-  // without a real factor lam that the pass's last op applies for it.
+  // Scaled forms (see qtile_scale_rotations): the op without a real factor
+  // lam that the pass's last op applies for it.
   // k = re(m0); re(m1) = 1 selects the first form, 0 the second.
   Q_ROTS = 10,    // [[1, -k], [k, 1]] or [[k, -1], [1, k]]
   Q_XLS = 11,     // [[1, ik], [ik, 1]] or [[k, i], [i, k]]
@@ -115,8 +115,7 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
 // common entry out of each diagonal op so its unit entries drop out, and
 // folds the product of the factors into a diagonal op at the end of the
 // pass.  Each factor is at least 1/sqrt(2) in modulus, so the amplitudes
-// inside a pass grow by at most 2^(ops/2).  Compiled passes only: the
-// interpreting kernel does not take the scaled kinds.
+// inside a pass grow by at most 2^(ops/2).
 void qtile_scale_rotations(QProgram& qp);
 
 struct QTileLaunch {
