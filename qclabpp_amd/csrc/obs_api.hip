@@ -397,6 +397,30 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
     }
     close(P.op_end);
   }
+  // barriers between ops: needed where the thread that owns an element
+  // changes (see TGen::sync), and before a separate inner-product sweep
+  auto owner_map = [](const TOp& t) {  // -1: thread e % 256 owns element e
+    constexpr int kOwnBits = 8;      // log2(kTileThreads)
+    switch (t.kind) {
+      case T_DENSE1:
+      case T_CDENSE1:
+        return t.a >= kOwnBits ? -1 : t.a;
+      case T_DENSE2:
+      case T_SWAP: {
+        const int lo = std::min(t.a, t.b), hi = std::max(t.a, t.b);
+        return lo >= kOwnBits ? -1 : 1000 + 64 * lo + hi;
+      }
+      default:
+        return -1;
+    }
+  };
+  for (const TPass& R : runs)
+    for (int j = R.op_begin; j < R.op_end; ++j) {
+      const bool last = j + 1 == R.op_end;
+      const int next = last ? -1 : owner_map(tops[j + 1]);
+      const bool sweep_next = !last && gens[j + 1].row >= 0 && !gens[j + 1].fused;
+      gens[j].sync = owner_map(tops[j]) != next || sweep_next;
+    }
   DevArray<TOp> d_tops;
   DevArray<TGen> d_gens;
   QSV_TRY(d_tops.upload(tops));

@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kTileThreads) void k_adjoint_tiles(C* __restrict__ 
           acc[g.row * 4 + wave].x += sr;
           acc[g.row * 4 + wave].y += si;
         }
-        __syncthreads();
+        if (g.sync) __syncthreads();  // block-uniform
         continue;
       }
       if (g.row >= 0) {  // block-uniform
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kTileThreads) void k_adjoint_tiles(C* __restrict__ 
       }
       GenAcc none{false, 0, 0u, 0u, 0u, 0u, 0.0, 0.0};
       apply2_top<C>(ops[o], Lf, Ll, size, base, none);
-      __syncthreads();
+      if (g.sync) __syncthreads();  // block-uniform
     }
     for (uint32_t e = threadIdx.x, k = 0; e < size; e += kTileThreads, ++k) {
       uint64_t off = lane_off;

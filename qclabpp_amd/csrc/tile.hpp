@@ -73,10 +73,17 @@ int compile_tiled(const std::vector<KOp>& ops, int n, int t, bool merge1q, int c
 // op skips lie outside the generator's projector, so the inner product is
 // accumulated inside the op's own loop from the values before the update;
 // otherwise it takes a sweep of its own before the op.
+// `sync`: a barrier must follow the op, because the next op (or the store
+// of the tile) has threads read elements that other threads wrote.  An op's
+// elements belong to threads by its loop: diagonal ops, and pair or quad ops
+// whose bits are all at local bit 8 or above, give thread tid the elements
+// e with e % 256 == tid, as the tile's load and store do.
 struct TGen {
   int32_t row;
   uint32_t floc, zloc, ploc, pvloc;
   int32_t fused;
+  int32_t sync;
+  int32_t pad;
   uint64_t zout, pout, pvout;
 };
 constexpr int kAdjRowsMax = 128;  // result rows per pass (LDS accumulators)
