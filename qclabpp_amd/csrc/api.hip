@@ -227,6 +227,7 @@ int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t 
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, lowered[i]));
   QProgram qp;
   QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qp, jit_tile_bits(dtype, n), true));
+  qtile_scale_rotations(qp);  // as Program::build does
   size_t total = 0;
   for (size_t i = 0; i < qp.passes.size(); ++i) {
     std::vector<char> code;

@@ -13,7 +13,7 @@ from qclabpp_amd import workloads as W
 from tests.test_gpu_tiled import every_kind
 
 
-def _check(n, circ, dtype):
+def _compile(n, circ, dtype):
     if not _lib.LIB_PATH.exists():
         pytest.skip("engine library not built")
     arr, cnt = G.records(circ)
@@ -24,7 +24,32 @@ def _check(n, circ, dtype):
         pytest.skip("hiprtc not loadable here")
     _lib.check(st)
     assert npass.value >= 1 and size.value > 0
-    return npass.value
+    return npass.value, size.value
+
+
+def _check(n, circ, dtype):
+    return _compile(n, circ, dtype)[0]
+
+
+@pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
+def test_generic_angles_share_compiled_code(dtype):
+    """Angles are data: two circuits of one structure whose generic angles
+    differ, including which of cos and sin is larger (the two scaled
+    rotation forms), compile to passes of the same number and code size."""
+    import random
+
+    def circ(seed):
+        r = random.Random(seed)
+        out = []
+        for layer in range(6):
+            for q in range(14):
+                ctor = (G.RotationX, G.RotationY, G.RotationZ)[(q + layer) % 3]
+                out.append(ctor(q, r.uniform(0.1, 3.0)))
+            for i in range(layer % 2, 13, 2):
+                out.append(G.CX(i, i + 1))
+        return out
+
+    assert _compile(14, circ(1), dtype) == _compile(14, circ(2), dtype)
 
 
 @pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
