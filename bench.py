@@ -504,10 +504,18 @@ def main():
         "wall_s": wall,
     }
     if best == "tiled_jit":
+        # the same launches against the FP64 VALU: the lane operations (FMA
+        # and multiply) their generated code holds, over the same mean time;
+        # 78.6 TFLOP/s counts an FMA as two, so the issue peak is 39.3e12
+        lane_ops = ji["fp_lane_ops"] / npass
         out["roofline"] = {"bound": "hbm", "achieved": gbps_pass, "peak": PEAK_HBM, "unit": "GB/s",
                            "frac": gbps_pass / PEAK_HBM, "traffic": measured_traffic(n, "qtp_*"),
                            "kernel": "qtp_* (compiled tiled passes, mean per launch)",
-                           "alg_bytes_per_launch": 2 * S}
+                           "alg_bytes_per_launch": 2 * S,
+                           "fp64_valu": {"lane_ops_per_launch": lane_ops,
+                                         "achieved": lane_ops / (ms_pass / 1e3) / 1e12,
+                                         "peak": PEAK_FP64 / 2, "unit": "1e12 lane-ops/s",
+                                         "frac": lane_ops / (ms_pass / 1e3) / 1e12 / (PEAK_FP64 / 2)}}
     elif best == "tiled":
         out["roofline"] = {"bound": "hbm", "achieved": gbps_ipass, "peak": PEAK_HBM, "unit": "GB/s",
                            "frac": gbps_ipass / PEAK_HBM, "traffic": measured_traffic(n, "k_qtile<double>"),

@@ -200,6 +200,10 @@ typedef struct qsv_program_info {
   int64_t ntiled_phases; /* tiled programs: register phases of one run  */
   int64_t ntiled_relabel; /* compiled tiled passes that write their tile
                              with qubits moved to other address bits     */
+  double  fp_lane_ops; /* tiled programs: floating-point lane operations
+                          (FMA and multiply; FP64 for complex128, FP32 for
+                          complex64) of one run's gate arithmetic, as the
+                          compiled passes generate it                    */
 } qsv_program_info;
 
 int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,

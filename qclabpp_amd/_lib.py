@@ -83,6 +83,7 @@ class QsvProgramInfo(C.Structure):
         ("flops", C.c_double),
         ("ntiled_phases", C.c_int64),
         ("ntiled_relabel", C.c_int64),
+        ("fp_lane_ops", C.c_double),
     ]
 
 

@@ -200,6 +200,7 @@ void Program::info(qsv_program_info* out) const {
     for (const QPass& P : qt.passes)
       out->ntiled_relabel += std::memcmp(P.tbits, P.tbits_out, sizeof(P.tbits)) != 0 ? 1 : 0;
     out->alg_bytes = double(qt.passes.size()) * 2.0 * std::ldexp(1.0, n) * eb;
+    out->fp_lane_ops = qtile_fp_lane_ops(qt);
     return;
   }
   out->nlaunch = int64_t(ops.size());

@@ -663,6 +663,26 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
 
 }  // namespace
 
+double qtile_fp_lane_ops(const QProgram& qp) {
+  constexpr int NR = 1 << kQRegBits;
+  double per_thread = 0;
+  for (const QOp& o : qp.ops) {
+    switch (o.kind) {
+      case Q_ROTS:
+      case Q_XLS: per_thread += 4 * (NR / 2); break;
+      case Q_ROTP:
+      case Q_XLIKEP:
+      case Q_REALP: per_thread += 8 * (NR / 2); break;
+      case Q_PAIR: per_thread += 16 * (NR / 2); break;
+      case Q_CPAIR: per_thread += 16 * __builtin_popcount(o.act); break;
+      case Q_DIAGR: per_thread += 4 * __builtin_popcount(uint32_t(o.b)); break;
+      case Q_DENSE2: per_thread += 64 * (NR / 4); break;
+      default: break;
+    }
+  }
+  return per_thread * std::ldexp(1.0, qp.n - kQRegBits);
+}
+
 void qtile_scale_rotations(QProgram& qp) {
   constexpr int NR = 1 << kQRegBits;
   std::vector<QOp> ops;

@@ -118,6 +118,13 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
 // inside a pass grow by at most 2^(ops/2).
 void qtile_scale_rotations(QProgram& qp);
 
+// Floating-point lane operations (FMA and multiply) of one run's gate
+// arithmetic as the compiled passes generate it: per op and thread, 4 per
+// register pair of a scaled rotation, 8 of a real 2x2, 16 of a complex one,
+// 4 per register of a diagonal and 64 per register quad of a 4x4; times
+// the 2^n / 16 threads of a pass.
+double qtile_fp_lane_ops(const QProgram& qp);
+
 struct QTileLaunch {
   void* psi;
   const QPass* pass;      // device

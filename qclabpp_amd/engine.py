@@ -182,7 +182,7 @@ class Program:
         return {"ngates": inf.ngates, "nlaunch": inf.nlaunch, "nplain": inf.nplain,
                 "ndiag": inf.ndiag, "ndense": list(inf.ndense), "alg_bytes": inf.alg_bytes,
                 "flops": inf.flops, "tiled_phases": inf.ntiled_phases,
-                "tiled_relabel": inf.ntiled_relabel}
+                "tiled_relabel": inf.ntiled_relabel, "fp_lane_ops": inf.fp_lane_ops}
 
     def launches(self):
         """Every launch of one run, in order: (qubits, diag, coefficients) with
