@@ -685,6 +685,10 @@ double qtile_fp_lane_ops(const QProgram& qp) {
 
 void qtile_scale_rotations(QProgram& qp) {
   constexpr int NR = 1 << kQRegBits;
+  // the factor taken out so far stays within [2^-20, 2^20] in modulus, so
+  // the stored amplitudes stay within 2^20 of their true scale however many
+  // gates a pass holds (a deep circuit on a small state is one pass)
+  constexpr double kMinLam = 1.0 / (1 << 20);
   std::vector<QOp> ops;
   ops.reserve(qp.ops.size() + qp.passes.size());
   for (const QPass& P : qp.passes) {
@@ -698,7 +702,18 @@ void qtile_scale_rotations(QProgram& qp) {
         // unitary ones only (c^2 + s^2 = 1), which bound the growth
         const double c = x.m[0], sn = x.kind == Q_ROTP ? x.m[4] : x.m[3];
         const bool unit = std::abs(c * c + sn * sn - 1.0) < 1e-12;
-        if (unit && (x.kind == Q_ROTP || (x.kind == Q_XLIKEP && x.m[0] == x.m[6] && x.m[3] == x.m[5]))) {
+        if (unit && (x.kind == Q_ROTP || x.kind == Q_XLIKEP) && std::abs(lam) < kMinLam) {
+          // the factors taken out so far shrink the stored amplitudes' scale
+          // too far: this op, times lam, becomes a complex 2x2 that puts the
+          // stored amplitudes back at their true scale
+          const cd e[4] = {cd(x.m[0]),
+                           x.kind == Q_ROTP ? cd(x.m[2]) : cd(0.0, x.m[3]),
+                           x.kind == Q_ROTP ? cd(x.m[4]) : cd(0.0, x.m[5]),
+                           cd(x.m[6])};
+          x.kind = Q_PAIR;
+          for (int i = 0; i < 4; ++i) set_c(x, i, lam * e[i]);
+          lam = 1.0;
+        } else if (unit && (x.kind == Q_ROTP || (x.kind == Q_XLIKEP && x.m[0] == x.m[6] && x.m[3] == x.m[5]))) {
           const bool first = std::abs(c) >= std::abs(sn);
           const double f = first ? c : sn;
           x.kind = x.kind == Q_ROTP ? Q_ROTS : Q_XLS;
@@ -717,7 +732,8 @@ void qtile_scale_rotations(QProgram& qp) {
             if (cnt > nbest) best = p, nbest = cnt;
             nunit += e[p] == cd(1.0);
           }
-          if (nbest > nunit) {
+          const double lf = std::abs(lam * e[best]);
+          if (nbest > nunit && lf >= kMinLam && lf <= 1.0 / kMinLam) {
             const cd f = e[best];
             x.b = 0;
             for (int p = 0; p < NR; ++p) {

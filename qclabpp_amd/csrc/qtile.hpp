@@ -114,8 +114,10 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
 // is max(|c|, |s|) times a matrix with a unit entry pair), factors the most
 // common entry out of each diagonal op so its unit entries drop out, and
 // folds the product of the factors into a diagonal op at the end of the
-// pass.  Each factor is at least 1/sqrt(2) in modulus, so the amplitudes
-// inside a pass grow by at most 2^(ops/2).
+// pass.  Each factor is at least 1/sqrt(2) in modulus; once their product
+// falls below 2^-20, the next rotation is kept as a complex 2x2 times that
+// product, which resets it, so the stored amplitudes stay within 2^20 of
+// their true scale in passes of any length.
 void qtile_scale_rotations(QProgram& qp);
 
 // Floating-point lane operations (FMA and multiply) of one run's gate

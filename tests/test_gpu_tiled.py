@@ -66,6 +66,30 @@ def test_tiled_every_kind_c64(n, form):
     assert np.abs(got - want).max() < 5e-5
 
 
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+def test_tiled_deep_pass_keeps_scale(dt):
+    """5000 rotations on a 12-qubit state are one pass of the interpreted
+    kernel (its tile is the whole state).  The rotations run in scaled forms
+    whose factors multiply up over the pass; the pass keeps its stored
+    amplitudes within 2^20 of their true scale, so neither precision
+    overflows."""
+    print(f"tiled deep pass {dt}", flush=True)
+    n = 12
+    rs = np.random.default_rng(31)
+    circ = []
+    for i in range(5000):
+        q = int(rs.integers(n))
+        circ.append((G.RotationX, G.RotationY)[i % 2](q, float(rs.uniform(0, 2 * np.pi))))
+        if i % 50 == 49:
+            circ.append(G.CX(q, (q + 1) % n))
+    psi = random_state(n, 8, np.complex128)
+    want = oracle.apply(n, circ, psi)
+    cdt = np.complex128 if dt == "c128" else np.complex64
+    got = run(E.Program(n, circ, fuse_k=E.TILED, dtype=dt), psi.astype(cdt))
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - want).max() < (1e-10 if dt == "c128" else 1e-4)
+
+
 @pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("n", [16, 20])
 def test_tiled_brickwork_and_ansatz(n, form):
