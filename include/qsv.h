@@ -210,12 +210,15 @@ int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, i
                        int diag_k, qsv_program** out);
 int qsv_program_run(const qsv_program* prog, void* psi, void* stream);
 /* Build check of QSV_TILED_JIT without a GPU: schedule the circuit into
- * tiled passes, generate every pass kernel and compile it for gfx950 with
- * hiprtc, without loading it.  *code_bytes (optional) receives the total
- * code-object size and *npasses (optional) the pass count.  No counterpart
- * in the reference; for build systems and CI hosts without a device.      */
+ * tiled passes, generate every pass kernel and, when code_bytes is given,
+ * compile it for gfx950 with hiprtc, without loading it.  *code_bytes
+ * (optional) receives the total code-object size, *npasses (optional) the
+ * pass count and *source_hash (optional) a 64-bit FNV-1a hash of the
+ * generated source of every pass, which is what the compiled-module cache
+ * is keyed by.  No counterpart in the reference; for build systems and CI
+ * hosts without a device.                                                  */
 int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
-                                size_t* code_bytes, int64_t* npasses);
+                                size_t* code_bytes, int64_t* npasses, uint64_t* source_hash);
 int qsv_program_info_get(const qsv_program* prog, qsv_program_info* out);
 /* Launch i of one run (0-based, launch order): the qubits it acts on
  * (ascending, k of them), and its operator as interleaved complex doubles --

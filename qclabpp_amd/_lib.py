@@ -143,7 +143,8 @@ SIGNATURES = {
     "qsv_host_apply_gates": (_I, [_P, _I, _I, _GP, _SZ]),
     "qsv_program_create": (_I, [_I, _I, _GP, _SZ, _I, _I, C.POINTER(_P)]),
     "qsv_program_run": (_I, [_P, _P, _P]),
-    "qsv_tiled_jit_compile_check": (_I, [_I, _I, _GP, _SZ, C.POINTER(_SZ), C.POINTER(C.c_int64)]),
+    "qsv_tiled_jit_compile_check": (_I, [_I, _I, _GP, _SZ, C.POINTER(_SZ), C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_uint64)]),
     "qsv_program_info_get": (_I, [_P, C.POINTER(QsvProgramInfo)]),
     "qsv_program_op": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP]),
     "qsv_program_destroy": (_I, [_P]),

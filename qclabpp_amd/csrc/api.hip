@@ -219,7 +219,7 @@ int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, i
 }
 
 int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
-                                size_t* code_bytes, int64_t* npasses) {
+                                size_t* code_bytes, int64_t* npasses, uint64_t* source_hash) {
   if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
   if (n < kQTileBits || n > 40) return fail(QSV_EINVAL, "tiled passes need n in [12, 40]");
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
@@ -229,12 +229,17 @@ int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t 
   QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qp, jit_tile_bits(dtype, n), true));
   qtile_scale_rotations(qp);  // as Program::build does
   size_t total = 0;
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over every pass's source
   for (size_t i = 0; i < qp.passes.size(); ++i) {
+    const std::string src = qtile_jit_source(qp, dtype == QSV_C128, i, i + 1);
+    for (unsigned char ch : src) h = (h ^ ch) * 1099511628211ull;
+    if (!code_bytes) continue;
     std::vector<char> code;
-    QSV_TRY(qtile_jit_compile(qtile_jit_source(qp, dtype == QSV_C128, i, i + 1), code));
+    QSV_TRY(qtile_jit_compile(src, code));
     total += code.size();
   }
   if (code_bytes) *code_bytes = total;
+  if (source_hash) *source_hash = h;
   if (npasses) *npasses = int64_t(qp.passes.size());
   return QSV_OK;
 }

@@ -685,14 +685,21 @@ double qtile_fp_lane_ops(const QProgram& qp) {
 
 void qtile_scale_rotations(QProgram& qp) {
   constexpr int NR = 1 << kQRegBits;
-  // the factor taken out so far stays within [2^-20, 2^20] in modulus, so
-  // the stored amplitudes stay within 2^20 of their true scale however many
-  // gates a pass holds (a deep circuit on a small state is one pass)
-  constexpr double kMinLam = 1.0 / (1 << 20);
+  // A scaled rotation takes out a factor of modulus in [2^-1/2, 1] and a
+  // unitary diagonal one of modulus 1, so after kMaxScaled scaled rotations
+  // the product is still at least 2^-20: the stored amplitudes stay within
+  // 2^20 of their true scale however many gates a pass holds (a deep
+  // circuit on a small state is one pass).  The next rotation then runs as
+  // a complex 2x2 times the product, which resets it.  The reset point is a
+  // count, not a test of the product, so the ops emitted (and the code
+  // compiled from them) depend on the circuit's structure, never on its
+  // angles.
+  constexpr int kMaxScaled = 40;
   std::vector<QOp> ops;
   ops.reserve(qp.ops.size() + qp.passes.size());
   for (const QPass& P : qp.passes) {
-    cd lam = 1.0;  // product of the factors taken out so far in this pass
+    cd lam = 1.0;     // product of the factors taken out so far in this pass
+    int nscaled = 0;  // scaled rotations since lam was last reset
     for (int ph = P.phase_begin; ph < P.phase_end; ++ph) {
       QPhase& Q = qp.phases[ph];
       const int begin = int(ops.size());
@@ -702,10 +709,12 @@ void qtile_scale_rotations(QProgram& qp) {
         // unitary ones only (c^2 + s^2 = 1), which bound the growth
         const double c = x.m[0], sn = x.kind == Q_ROTP ? x.m[4] : x.m[3];
         const bool unit = std::abs(c * c + sn * sn - 1.0) < 1e-12;
-        if (unit && (x.kind == Q_ROTP || x.kind == Q_XLIKEP) && std::abs(lam) < kMinLam) {
-          // the factors taken out so far shrink the stored amplitudes' scale
-          // too far: this op, times lam, becomes a complex 2x2 that puts the
-          // stored amplitudes back at their true scale
+        const bool scalable =
+            unit && (x.kind == Q_ROTP || (x.kind == Q_XLIKEP && x.m[0] == x.m[6] && x.m[3] == x.m[5]));
+        if (scalable && nscaled == kMaxScaled) {
+          // the factors taken out so far may have shrunk the stored
+          // amplitudes' scale by up to 2^20: this op, times lam, becomes a
+          // complex 2x2 that puts them back at their true scale
           const cd e[4] = {cd(x.m[0]),
                            x.kind == Q_ROTP ? cd(x.m[2]) : cd(0.0, x.m[3]),
                            x.kind == Q_ROTP ? cd(x.m[4]) : cd(0.0, x.m[5]),
@@ -713,7 +722,8 @@ void qtile_scale_rotations(QProgram& qp) {
           x.kind = Q_PAIR;
           for (int i = 0; i < 4; ++i) set_c(x, i, lam * e[i]);
           lam = 1.0;
-        } else if (unit && (x.kind == Q_ROTP || (x.kind == Q_XLIKEP && x.m[0] == x.m[6] && x.m[3] == x.m[5]))) {
+          nscaled = 0;
+        } else if (scalable) {
           const bool first = std::abs(c) >= std::abs(sn);
           const double f = first ? c : sn;
           x.kind = x.kind == Q_ROTP ? Q_ROTS : Q_XLS;
@@ -721,6 +731,7 @@ void qtile_scale_rotations(QProgram& qp) {
           set_c(x, 0, first ? sn / c : c / sn);
           set_c(x, 1, first ? 1.0 : 0.0);
           lam *= f;
+          ++nscaled;
         } else if (x.kind == Q_DIAGR) {
           // take out the most common entry (unit entries of registers outside b included)
           cd e[NR];
@@ -732,8 +743,10 @@ void qtile_scale_rotations(QProgram& qp) {
             if (cnt > nbest) best = p, nbest = cnt;
             nunit += e[p] == cd(1.0);
           }
-          const double lf = std::abs(lam * e[best]);
-          if (nbest > nunit && lf >= kMinLam && lf <= 1.0 / kMinLam) {
+          // only a unit-modulus entry is taken out, so lam's modulus (and
+          // the reset count above) is unaffected; the test is on the gate's
+          // own entries, which for a rotation's phases hold for every angle
+          if (nbest > nunit && std::abs(std::abs(e[best]) - 1.0) < 1e-12) {
             const cd f = e[best];
             x.b = 0;
             for (int p = 0; p < NR; ++p) {

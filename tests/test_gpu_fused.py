@@ -9,6 +9,7 @@ from qclabpp_amd import gates as G
 from qclabpp_amd import workloads as W
 from qclabpp_amd.rng import normal_state
 from tests.gatecases import random_state, random_unitary
+from tests.parity import C64, C128, assert_close
 
 pytestmark = pytest.mark.gpu
 
@@ -43,7 +44,7 @@ def test_programs_vs_oracle_c128(n, k):
     psi = random_state(n, 3, np.complex128)
     want = oracle.apply(n, circ, psi)
     got = run(E.Program(n, circ, fuse_k=k), psi)
-    assert np.abs(got - want).max() < 1e-12
+    assert_close(got, want, C128, f"n={n} k={k}")
 
 
 @pytest.mark.parametrize("k", [0, 4, 5])
@@ -53,7 +54,7 @@ def test_programs_c64(k):
     psi = random_state(n, 4, np.complex128)
     want = oracle.apply(n, circ, psi)
     got = run(E.Program(n, circ, fuse_k=k, dtype="c64"), psi.astype(np.complex64))
-    assert np.abs(got - want).max() < 5e-5
+    assert_close(got, want, C64, f"c64 k={k}")
 
 
 def test_every_block_qubit_layout():
@@ -71,7 +72,7 @@ def test_every_block_qubit_layout():
     want = oracle.apply(n, circ, psi)
     for k in (3, 4, 5):
         got = run(E.Program(n, circ, fuse_k=k), psi)
-        assert np.abs(got - want).max() < 1e-12, k
+        assert_close(got, want, C128, f"k={k}")
 
 
 def test_fused_qft20_is_dft():
@@ -80,4 +81,4 @@ def test_fused_qft20_is_dft():
     want = np.fft.ifft(x) * np.sqrt(1 << n)
     for k in (4, 5):
         got = run(E.Program(n, W.qft(n), fuse_k=k), x)
-        assert np.abs(got - want).max() < 1e-12
+        assert_close(got, want, C128, f"QFT k={k}")

@@ -4,6 +4,7 @@ generation and hiprtc here, so a generator error fails the CPU suite."""
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import pytest
 
@@ -19,12 +20,26 @@ def _compile(n, circ, dtype):
     arr, cnt = G.records(circ)
     size = C.c_size_t()
     npass = C.c_int64()
-    st = _lib.lib().qsv_tiled_jit_compile_check(n, dtype, arr, cnt, C.byref(size), C.byref(npass))
+    st = _lib.lib().qsv_tiled_jit_compile_check(n, dtype, arr, cnt, C.byref(size), C.byref(npass),
+                                                None)
     if st == _lib.QSV_EUNSUP and b"hiprtc" in _lib.lib().qsv_last_error():
         pytest.skip("hiprtc not loadable here")
     _lib.check(st)
     assert npass.value >= 1 and size.value > 0
     return npass.value, size.value
+
+
+def _source_hash(n, circ, dtype):
+    """(pass count, hash of the generated source of every pass), without
+    compiling: the compiled-module cache is keyed by that source."""
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("engine library not built")
+    arr, cnt = G.records(circ)
+    npass = C.c_int64()
+    h = C.c_uint64()
+    _lib.check(_lib.lib().qsv_tiled_jit_compile_check(n, dtype, arr, cnt, None, C.byref(npass),
+                                                      C.byref(h)))
+    return npass.value, h.value
 
 
 def _check(n, circ, dtype):
@@ -50,6 +65,37 @@ def test_generic_angles_share_compiled_code(dtype):
         return out
 
     assert _compile(14, circ(1), dtype) == _compile(14, circ(2), dtype)
+    assert _source_hash(14, circ(1), dtype) == _source_hash(14, circ(2), dtype)
+
+
+@pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
+def test_deep_pass_source_is_angle_independent(dtype):
+    """A deep circuit on 12 qubits is one pass whose scaled rotations reset
+    their factored-out product many times.  The reset points depend on the
+    structure only, so every angle set -- small and large angles, cos or sin
+    the larger, angles that take the product to its bound fastest -- gives
+    the same generated source and shares one compiled module."""
+    import random
+
+    n = 12
+
+    def circ(angle):
+        out = []
+        for layer in range(14):
+            for q in range(n):
+                out.append((G.RotationX, G.RotationY)[(q + layer) % 2](q, angle(layer, q)))
+            for i in range(layer % 2, n - 1, 2):
+                out.append(G.CX(i, i + 1))
+        return out
+
+    r = random.Random(5)
+    sets = [lambda l, q: r.uniform(0, 2 * math.pi),
+            lambda l, q: 0.01 + 0.001 * q,                  # |cos| ~ 1: factors near 1
+            lambda l, q: math.pi / 2 + 0.001 * (l + q),     # |cos| ~ |sin|: factors near 2^-1/2
+            lambda l, q: 3.0 - 0.01 * l]
+    hashes = {_source_hash(n, circ(a), dtype) for a in sets}
+    assert len(hashes) == 1, hashes
+    assert next(iter(hashes))[0] == 1  # the whole circuit is one pass
 
 
 @pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
@@ -68,8 +114,8 @@ def test_bad_arguments_are_rejected():
     if not _lib.LIB_PATH.exists():
         pytest.skip("engine library not built")
     lib = _lib.lib()
-    assert lib.qsv_tiled_jit_compile_check(8, _lib.QSV_C128, None, 0, None, None) == _lib.QSV_EINVAL
-    assert lib.qsv_tiled_jit_compile_check(14, 7, None, 0, None, None) == _lib.QSV_EDTYPE
+    assert lib.qsv_tiled_jit_compile_check(8, _lib.QSV_C128, None, 0, None, None, None) == _lib.QSV_EINVAL
+    assert lib.qsv_tiled_jit_compile_check(14, 7, None, 0, None, None, None) == _lib.QSV_EDTYPE
 
 
 @pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
