@@ -47,8 +47,9 @@ def parse():
     ap.add_argument("--adjoint-layers", type=int, default=10)
     ap.add_argument("--adjoint-big-n", type=int, default=30,
                     help="qubits of the memory-scale adjoint run (three 2^n states; 0 skips it)")
-    ap.add_argument("--ps-params", type=int, default=1,
-                    help="parameters checked against the CPU parameter-shift gradient")
+    ap.add_argument("--ps-params", type=int, default=6,
+                    help="parameters checked against the CPU parameter-shift gradient "
+                         "(a seeded pick spread evenly over RY, RZ and RZZ)")
     ap.add_argument("--batch", type=int, default=512,
                     help="circuits of the batched HEA sweep (0 skips it)")
     ap.add_argument("--batch-runs", type=int, default=5)
@@ -108,9 +109,12 @@ def adjoint_leg(args):
     """E and dE/dtheta of the TFIM chain H = sum Z_i Z_{i+1} + 0.7 sum X_i for
     the RY/RZ/RZZ ansatz (n=24, 10 layers: 710 parameters): the HIP adjoint
     sweep, timed against central finite differences (h=1e-6) over all
-    parameters on the same GPU, each difference two fused forward
-    simulations plus FP64 expectation values.  A few parameters are also
-    checked against the CPU parameter-shift gradient of the oracle."""
+    parameters on the same GPU, each difference two forward simulations in
+    the engine's fastest form (compiled tiled passes; the shifted circuits
+    share one schedule and one set of compiled kernels) plus FP64
+    expectation values.  A seeded subset of parameters over RY, RZ and RZZ
+    is also checked against the CPU parameter-shift gradient of the
+    oracle."""
     import numpy as np
     import torch
 
@@ -125,16 +129,16 @@ def adjoint_leg(args):
     H = O.tfim(n)
     psi = E.new_state(n)
     lam = torch.empty_like(psi)
-    O.adjoint_gradient(psi, lam, circ, H, fuse_k=5)  # warm-up
+    O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)  # warm-up (compiles the forward passes)
     E.set_basis(psi, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    e_adj, g_adj = O.adjoint_gradient(psi, lam, circ, H, fuse_k=5)
+    e_adj, g_adj = O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
     del psi, lam
     t0 = time.perf_counter()
-    g_fd = finite_difference(circ, H, theta, n, h=1e-6, fuse_k=5)
+    g_fd = finite_difference(circ, H, theta, n, h=1e-6, fuse_k=E.AUTO)
     torch.cuda.synchronize()
     t_fd = time.perf_counter() - t0
     out = {"workload": f"tfim ansatz n={n} layers={L} params={theta.size} complex128",
@@ -143,10 +147,20 @@ def adjoint_leg(args):
            "max_abs_grad": float(np.abs(g_adj).max())}
     if args.ps_params > 0 and not args.no_cpu:
         import oracle
-        which = list(range(0, theta.size, max(1, theta.size // args.ps_params)))[:args.ps_params]
+        rs = np.random.default_rng(20261205)
+        by_kind = {}
+        for g in circ:
+            if g.param >= 0:
+                by_kind.setdefault(g.name, []).append(g.param)
+        kinds = sorted(by_kind)
+        which = sorted(int(p) for i, k in enumerate(kinds)
+                       for p in rs.choice(by_kind[k], args.ps_params // len(kinds)
+                                          + (i < args.ps_params % len(kinds)), replace=False))
         t0 = time.perf_counter()
+        circ_kind = {g.param: g.name for g in circ if g.param >= 0}
         ps = oracle.param_shift(n, circ, H, theta, which)
-        out["param_shift_cpu"] = {"params": which, "s": time.perf_counter() - t0,
+        out["param_shift_cpu"] = {"params": which, "kinds": [circ_kind[p] for p in which],
+                                  "s": time.perf_counter() - t0,
                                   "max_abs_adjoint_minus_ps": float(np.abs(g_adj[which] - ps).max())}
     return out
 
@@ -327,6 +341,11 @@ def precision_leg(args):
     return out
 
 
+TRAFFIC_SOURCE = ("profiles/hbm_traffic_n30_c128.md: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch "
+                  "of a separate run of the same workload (a counter run cannot also be the timed run), "
+                  "each scaled by the same counter of a 16 GiB copy")
+
+
 def measured_traffic(n, kernel="k_fused_mfma<double,5>"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by a copy kernel), or None when
@@ -352,27 +371,72 @@ def timed(torch, fn, reps, stream):
     return e0.elapsed_time(e1) / reps  # ms
 
 
-def cpu_baseline(n, circ, count):
+def host_cores():
+    """(physical cores, logical CPUs) this process may run on: the CPUs of
+    its affinity mask, grouped by (package, core) from sysfs."""
+    cpus = sorted(os.sched_getaffinity(0))
+    phys = set()
+    for c in cpus:
+        base = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            phys.add((base.joinpath("physical_package_id").read_text().strip(),
+                      base.joinpath("core_id").read_text().strip()))
+        except OSError:
+            phys.add(("?", str(c)))
+    return len(phys), len(cpus)
+
+
+# Runs in a child process, so that the OpenMP runtime of the reference
+# library starts with the placement set in its environment.
+_CPU_CHILD = r"""
+import json, sys, time
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import oracle
+from qclabpp_amd import workloads as W
+n, count, depth = int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4])
+sample = W.brickwork(n, depth)[:count]
+if oracle.ref_available():
+    secs = oracle.ref_time_simulate_zero(n, sample)
+    kind, threads = "reference", oracle.ref_threads()
+else:
+    psi = np.zeros(1 << n, np.complex128)
+    psi[0] = 1
+    t0 = time.perf_counter()
+    oracle.apply(n, sample, psi)
+    secs = time.perf_counter() - t0
+    kind, threads = "port", None
+print(json.dumps({"secs": secs, "kind": kind, "threads": threads, "gates": len(sample)}))
+"""
+
+
+def cpu_baseline(n, depth, count):
     """QCLAB++'s own QCircuit::simulate (oracle/_ref, built from the
-    reference sources) on the first `count` gates, from |0...0>; the CPU
-    restatement when that library is absent."""
-    import oracle
-    from qclabpp_amd import gates as G
-    sample = circ[:count]
-    if oracle.ref_available():
-        secs = oracle.ref_time_simulate_zero(n, sample)
-        kind, cores = "reference", oracle.ref_threads()
-    else:
-        import numpy as np
-        psi = np.zeros(1 << n, np.complex128)
-        psi[0] = 1
-        t0 = time.perf_counter()
-        oracle.apply(n, sample, psi)
-        secs = time.perf_counter() - t0
-        kind, cores = "port", os.cpu_count()
-    return {"value": len(sample) / secs, "unit": "gate-passes/s", "cores": cores, "kind": kind,
-            "sample": f"first {len(sample)} gates of the n={n} brickwork (its first layer), "
-                      f"QCircuit::simulate on |0...0>, {secs:.2f} s"}
+    reference sources) on the first `count` gates (the first layer) of the
+    same brickwork, from |0...0>; the CPU restatement when that library is
+    absent.  One OpenMP thread per physical core of this process's CPUs
+    (at most OMP_NUM_THREADS), OMP_PROC_BIND=close, OMP_PLACES=cores."""
+    import subprocess
+    phys, logical = host_cores()
+    threads = min(phys, int(os.environ.get("OMP_NUM_THREADS", phys) or phys))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    res = subprocess.run([sys.executable, "-c", _CPU_CHILD, str(ROOT), str(n), str(count), str(depth)],
+                         capture_output=True, text=True, env=env, timeout=600)
+    if res.returncode != 0:
+        return {"error": res.stderr[-1000:]}
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    secs = r["secs"]
+    return {"value": r["gates"] / secs, "unit": "gate-passes/s", "cores": r["threads"] or threads,
+            "kind": r["kind"], "host_physical_cores": phys, "host_logical_cpus": logical,
+            "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"},
+            "sample": f"first {r['gates']} gates of the n={n} brickwork (its first layer), "
+                      f"QCircuit::simulate on |0...0>, {secs:.2f} s; the per-gate cost is uniform "
+                      f"(one full sweep each), so the rate stands for the whole circuit"}
+
+
+def progress(msg):
+    """A progress line on stderr (the JSON result alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -394,6 +458,7 @@ def main():
     S = (1 << n) * 16
     stream = torch.cuda.current_stream()
 
+    progress(f"creating programs for the n={n} depth-{depth} brickwork")
     t0 = time.perf_counter()
     jit = E.Program(n, circ, fuse_k=E.TILED_JIT)
     jit_s = time.perf_counter() - t0
@@ -413,6 +478,7 @@ def main():
             plain.run(psi)
     torch.cuda.synchronize()
 
+    progress("warm-up done; timing the compiled tiled passes")
     # ---- timed region: K steps of the compiled tiled program -----------------
     t_wall = time.perf_counter()
     ms_jit = timed(torch, lambda: jit.run(psi), args.steps, stream)
@@ -510,6 +576,7 @@ def main():
         lane_ops = ji["fp_lane_ops"] / npass
         out["roofline"] = {"bound": "hbm", "achieved": gbps_pass, "peak": PEAK_HBM, "unit": "GB/s",
                            "frac": gbps_pass / PEAK_HBM, "traffic": measured_traffic(n, "qtp_*"),
+                           "traffic_source": TRAFFIC_SOURCE,
                            "kernel": "qtp_* (compiled tiled passes, mean per launch)",
                            "alg_bytes_per_launch": 2 * S,
                            "fp64_valu": {"lane_ops_per_launch": lane_ops,
@@ -519,29 +586,39 @@ def main():
     elif best == "tiled":
         out["roofline"] = {"bound": "hbm", "achieved": gbps_ipass, "peak": PEAK_HBM, "unit": "GB/s",
                            "frac": gbps_ipass / PEAK_HBM, "traffic": measured_traffic(n, "k_qtile<double>"),
+                           "traffic_source": TRAFFIC_SOURCE,
                            "kernel": "k_qtile<double>", "alg_bytes_per_launch": 2 * S}
     else:
         out["roofline"] = {"bound": "hbm", "achieved": gbps_blk, "peak": PEAK_HBM, "unit": "GB/s",
                            "frac": gbps_blk / PEAK_HBM,
                            "traffic": measured_traffic(n, "k_fused_mfma<double,5>"),
+                           "traffic_source": TRAFFIC_SOURCE,
                            "kernel": "k_fused_mfma<double,5>", "alg_bytes_per_launch": 2 * S}
     del psi
     torch.cuda.empty_cache()
+    progress(f"main leg: {ms_best:.1f} ms per step ({best})")
     if args.host_entry:
+        progress("host-vector entry")
         out["host_entry"] = host_entry_leg(args, circ, ms_jit)
     if args.adjoint_n > 0:
+        progress(f"adjoint n={args.adjoint_n}")
         out["adjoint"] = adjoint_leg(args)
         torch.cuda.empty_cache()
     if args.adjoint_big_n > 0:
+        progress(f"adjoint n={args.adjoint_big_n}")
         out["adjoint_big"] = adjoint_big_leg(args)
         torch.cuda.empty_cache()
     if args.batch > 0:
+        progress(f"batch of {args.batch}")
         out["batch"] = batch_leg(args)
         torch.cuda.empty_cache()
     if args.precision_n > 0 or args.precision_big_n > 0:
+        progress("precision")
         out["precision"] = precision_leg(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n, circ, args.cpu_gates)
+        progress("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(n, depth, args.cpu_gates)
+    progress("done")
     print(json.dumps(out))
     return 0
 

@@ -177,15 +177,18 @@ int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates,
  * qubits use fuse_k = 5 instead).  Coefficients live on the device for the
  * program's lifetime; running it allocates nothing.                       */
 #define QSV_TILED (-1)
-/* fuse_k == QSV_AUTO picks QSV_TILED from 22 qubits up (at least 1024 tiles
- * to spread over the GPU) and fuse_k = 5 below.                          */
+/* fuse_k == QSV_AUTO picks QSV_TILED_JIT from 22 qubits up (at least 1024
+ * tiles to spread over the GPU), QSV_TILED there where hiprtc cannot be
+ * loaded, and fuse_k = 5 below 22 qubits.                                */
 #define QSV_AUTO  (-2)
 /* fuse_k == QSV_TILED_JIT runs the same tiled passes as QSV_TILED, each pass
  * compiled at program creation into a kernel of its own (hiprtc, gfx950)
- * with the gate structure as constants: faster passes for a compile of
- * about a second, done once per circuit structure and process (programs
- * whose gates differ only in angles share the compiled passes).  Creation
- * fails with QSV_EUNSUP where hiprtc cannot be loaded.                    */
+ * with the gate structure as constants: faster passes for a compile of a
+ * few seconds, done once per circuit structure.  Programs whose gates
+ * differ only in angles share the schedule and the compiled passes (both
+ * cached per process; the code objects also on disk, see
+ * qsv_cache_stats).  Creation fails with QSV_EUNSUP where hiprtc cannot be
+ * loaded.                                                                 */
 #define QSV_TILED_JIT (-3)
 typedef struct qsv_program qsv_program;
 
@@ -333,6 +336,22 @@ int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_
 int qsv_dev_simulate(void* psi, int n, int dtype, const qsv_gate* gates, size_t ngates,
                      int fuse_k, void* stream);
 int qsv_release_cache(void);
+
+/* Counters of the compile caches since the library was loaded, so callers
+ * (and tests) can see that a repeated simulate of one circuit structure
+ * skipped the schedule search and the compile.  No counterpart in the
+ * reference.
+ *   schedule_hits/misses: tiled schedules whose tie rule was known for the
+ *     circuit structure (one rule run) / searched over every rule;
+ *   module_hits: compiled pass groups found in the process-wide cache;
+ *   module_disk_hits: loaded from the on-disk cache
+ *     ($QSV_JIT_CACHE_DIR, else ~/.cache/qclabpp_amd/jit; "" disables it);
+ *   module_compiles: compiled with hiprtc.                                 */
+typedef struct qsv_cache_stats {
+  int64_t schedule_hits, schedule_misses;
+  int64_t module_hits, module_disk_hits, module_compiles;
+} qsv_cache_stats;
+int qsv_cache_stats_get(qsv_cache_stats* out);
 
 /* ---- unitaries (QCircuit::matrix, nrmF) -----------------------------------
  * Replace QCircuit<T>::matrix() (include/qclab/QCircuit.hpp:121-127, the

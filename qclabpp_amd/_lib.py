@@ -70,6 +70,21 @@ class QsvTuneResult(C.Structure):
     _fields_ = [("opts", QsvBatchOpts), ("ms", C.c_double), ("max_abs_diff", C.c_double)]
 
 
+class QsvCacheStats(C.Structure):
+    """Mirror of `qsv_cache_stats` (include/qsv.h)."""
+
+    _fields_ = [("schedule_hits", C.c_int64), ("schedule_misses", C.c_int64),
+                ("module_hits", C.c_int64), ("module_disk_hits", C.c_int64),
+                ("module_compiles", C.c_int64)]
+
+
+def cache_stats() -> dict:
+    """The compile caches' counters (qsv_cache_stats_get)."""
+    st = QsvCacheStats()
+    check(lib().qsv_cache_stats_get(C.byref(st)))
+    return {k: getattr(st, k) for k, _ in QsvCacheStats._fields_}
+
+
 class QsvProgramInfo(C.Structure):
     """Mirror of `qsv_program_info` (include/qsv.h)."""
 
@@ -149,6 +164,7 @@ SIGNATURES = {
     "qsv_program_op": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP]),
     "qsv_program_destroy": (_I, [_P]),
     "qsv_release_cache": (_I, []),
+    "qsv_cache_stats_get": (_I, [C.POINTER(QsvCacheStats)]),
     "qsv_host_simulate": (_I, [_P, _I, _I, _GP, _SZ, _I]),
     "qsv_dev_simulate": (_I, [_P, _I, _I, _GP, _SZ, _I, _P]),
     "qsv_dev_expectation": (_I, [_P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _DP, _DP, _P]),

@@ -19,10 +19,6 @@ def _torch():
     return torch
 
 
-class _AdjointEnergy:
-    pass
-
-
 def _make_function():
     torch = _torch()
 
@@ -49,7 +45,7 @@ def _make_function():
 _FN = None
 
 
-def energy(theta, circuit, H, n: int, dtype: str = "c128", fuse_k: int = 5):
+def energy(theta, circuit, H, n: int, dtype: str = "c128", fuse_k: int = E.AUTO):
     """Differentiable E(theta); `circuit` is a gate list whose parametrised
     gates carry `param` indices into theta."""
     global _FN
@@ -58,11 +54,15 @@ def energy(theta, circuit, H, n: int, dtype: str = "c128", fuse_k: int = 5):
     return _FN.apply(theta, circuit, H, n, dtype, fuse_k)
 
 
-def finite_difference(circuit, H, theta, n: int, h: float = 1e-6, fuse_k: int = 5,
+def finite_difference(circuit, H, theta, n: int, h: float = 1e-6, fuse_k: int = E.AUTO,
                       dtype: str = "c128", which=None):
     """Central differences [E(theta + h e_p) - E(theta - h e_p)] / (2h) on
-    the GPU, each E a fused forward simulation from |0...0> plus an FP64
-    expectation value."""
+    the GPU, each E a forward simulation from |0...0> plus an FP64
+    expectation value.  The default form is the engine's fastest (AUTO:
+    compiled tiled passes from 22 qubits up, fused blocks below); every
+    shifted circuit has the structure of the first, so its program reuses
+    the cached schedule and compiled passes and only its coefficients are
+    new."""
     torch = _torch()
     theta = np.asarray(theta, dtype=np.float64)
     cdt = torch.complex128 if dtype == "c128" else torch.complex64

@@ -415,6 +415,13 @@ int qsv_circuit_nrmf(int k, int dtype, const qsv_gate* gates1, size_t ngates1, c
   return rc;
 }
 
+int qsv_cache_stats_get(qsv_cache_stats* out) {
+  if (!out) return fail(QSV_EINVAL, "null argument");
+  qtile_schedule_cache_stats(&out->schedule_hits, &out->schedule_misses);
+  qtile_jit_cache_stats(&out->module_hits, &out->module_disk_hits, &out->module_compiles);
+  return QSV_OK;
+}
+
 int qsv_release_cache(void) {
   std::lock_guard<std::mutex> lk(g_cache.mu);
   if (g_cache.ptr) QSV_CHECK_HIP(hipFree(g_cache.ptr));

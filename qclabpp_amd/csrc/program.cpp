@@ -71,7 +71,10 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
   ngates_in = ngates;
   std::vector<KOp> lowered(ngates);
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, lowered[i]));
-  if (fuse_k == QSV_AUTO) fuse_k = n >= kQTileBits + 10 ? QSV_TILED : kMaxFuse;
+  // the compiled passes are the fastest form from 2^10 tiles up; they need
+  // hiprtc, without it the interpreting kernel runs the same passes
+  if (fuse_k == QSV_AUTO)
+    fuse_k = n >= kQTileBits + 10 ? (qtile_jit_available() ? QSV_TILED_JIT : QSV_TILED) : kMaxFuse;
   if ((fuse_k == QSV_TILED || fuse_k == QSV_TILED_JIT) && n < kQTileBits)
     fuse_k = kMaxFuse;  // one tile would be the whole state
   if (fuse_k == QSV_TILED || fuse_k == QSV_TILED_JIT) {

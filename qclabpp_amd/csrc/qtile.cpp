@@ -6,9 +6,12 @@
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <mutex>
 #include <set>
 #include <string>
 #include <thread>
+#include <unordered_map>
 
 namespace qsv {
 
@@ -781,6 +784,39 @@ void qtile_scale_rotations(QProgram& qp) {
   qp.ops.swap(ops);
 }
 
+namespace {
+
+// The winning tie rule per circuit structure.  The key holds what the
+// schedule search looks at -- sizes, op kinds, qubits, control states --
+// and no coefficients, so circuits that differ only in angles share it.
+std::mutex g_sched_mu;
+std::unordered_map<std::string, int> g_sched;
+constexpr size_t kSchedCacheMax = 1024;
+std::atomic<int64_t> g_sched_hits{0}, g_sched_misses{0};
+
+std::string schedule_key(const std::vector<KOp>& in, int n, int line_bits, int tile_bits,
+                         bool relabel) {
+  std::string k;
+  k.reserve(16 + in.size() * 8);
+  auto put = [&](int v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
+  put(n);
+  put(line_bits);
+  put(tile_bits);
+  put(relabel);
+  for (const KOp& op : in) {
+    put(op.kind | op.nq << 8 | (op.cs & 1) << 12);
+    for (int i = 0; i < op.nq; ++i) k.push_back(char(op.q[i]));
+  }
+  return k;
+}
+
+}  // namespace
+
+void qtile_schedule_cache_stats(int64_t* hits, int64_t* misses) {
+  if (hits) *hits = g_sched_hits.load();
+  if (misses) *misses = g_sched_misses.load();
+}
+
 int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out, int tile_bits,
                   bool relabel) {
   if (tile_bits < kQTileBits || tile_bits > kQTileBitsMax)
@@ -805,6 +841,23 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
   const int tries = int(std::clamp<size_t>(size_t(128) * 1000 / gates, 8, 128));
   const int variants = relabel ? 2 : 1;
   const int total = tries * variants;
+  const std::string key = schedule_key(in, n, line_bits, tile_bits, relabel);
+  int known = -1;
+  {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    auto it = g_sched.find(key);
+    if (it != g_sched.end()) known = it->second;
+  }
+  if (known >= 0 && known < total) {
+    QProgram p;
+    if (compile_qtile_with(in, n, line_bits, tile_bits, known % tries, known >= tries, p) == QSV_OK) {
+      ++g_sched_hits;
+      out = std::move(p);
+      out.tile_bits = tile_bits;
+      return QSV_OK;
+    }
+  }
+  ++g_sched_misses;
   std::vector<QProgram> cand(total);
   std::vector<int> rc(total, QSV_OK);
   std::vector<std::string> err(total);
@@ -849,6 +902,11 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
   }
   out = std::move(cand[best]);
   out.tile_bits = tile_bits;
+  {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    if (g_sched.size() >= kSchedCacheMax) g_sched.clear();
+    g_sched[key] = best;
+  }
   return QSV_OK;
 }
 

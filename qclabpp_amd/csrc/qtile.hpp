@@ -114,11 +114,16 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
 // is max(|c|, |s|) times a matrix with a unit entry pair), factors the most
 // common entry out of each diagonal op so its unit entries drop out, and
 // folds the product of the factors into a diagonal op at the end of the
-// pass.  Each factor is at least 1/sqrt(2) in modulus; once their product
-// falls below 2^-20, the next rotation is kept as a complex 2x2 times that
-// product, which resets it, so the stored amplitudes stay within 2^20 of
-// their true scale in passes of any length.
+// pass.  Each factor is at least 1/sqrt(2) in modulus; after 40 of them the
+// next rotation is kept as a complex 2x2 times their product, which resets
+// it, so the stored amplitudes stay within 2^20 of their true scale in
+// passes of any length.  The reset points depend on the structure only.
 void qtile_scale_rotations(QProgram& qp);
+
+// Hits and misses of the schedule cache: compile_qtile remembers, per
+// circuit structure (gate kinds, qubits and control states, not angles),
+// which tie rule won, and on a hit runs that rule alone.
+void qtile_schedule_cache_stats(int64_t* hits, int64_t* misses);
 
 // Floating-point lane operations (FMA and multiply) of one run's gate
 // arithmetic as the compiled passes generate it: per op and thread, 4 per

@@ -7,9 +7,14 @@
 #include <dlfcn.h>
 #include <hip/hiprtc.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cstdio>
+#include <fstream>
 #include <map>
 #include <mutex>
 #include <cstring>
@@ -502,8 +507,73 @@ struct CacheKey {
 std::mutex g_cache_mu;
 std::map<CacheKey, std::shared_ptr<QJitModule>> g_cache;
 constexpr size_t kCacheMax = 256;  // modules kept for reuse; programs hold their own references
+std::atomic<int64_t> g_mod_hits{0}, g_disk_hits{0}, g_compiles{0};
+
+// ---- code objects on disk ------------------------------------------------
+// A compiled group of passes is also kept as a file named by two 64-bit
+// hashes of its source (and the source length), so a new process with a
+// circuit of a known structure loads it instead of compiling.  The directory
+// is $QSV_JIT_CACHE_DIR, else $XDG_CACHE_HOME/qclabpp_amd/jit, else
+// $HOME/.cache/qclabpp_amd/jit; QSV_JIT_CACHE_DIR set to an empty string
+// turns it off.  Files are written to a temporary name and renamed, so a
+// reader never sees a partial one; a file that fails to load is compiled
+// again and replaced.
+
+std::string disk_cache_dir() {
+  if (const char* e = std::getenv("QSV_JIT_CACHE_DIR")) return e;
+  if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) return std::string(x) + "/qclabpp_amd/jit";
+  if (const char* h = std::getenv("HOME"); h && *h) return std::string(h) + "/.cache/qclabpp_amd/jit";
+  return "";
+}
+
+std::string disk_cache_file(const std::string& dir, const std::string& src) {
+  uint64_t a = 1469598103934665603ull, b = 0x9e3779b97f4a7c15ull;
+  for (unsigned char c : src) {
+    a = (a ^ c) * 1099511628211ull;
+    b = (b + c) * 0xff51afd7ed558ccdull;
+    b ^= b >> 29;
+  }
+  char name[80];
+  std::snprintf(name, sizeof name, "/qtp_%016llx%016llx_%zu.co", (unsigned long long)a,
+                (unsigned long long)b, src.size());
+  return dir + name;
+}
+
+bool disk_cache_read(const std::string& path, std::vector<char>& code) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+  return !code.empty();
+}
+
+void disk_cache_write(const std::string& dir, const std::string& path, const std::vector<char>& code) {
+  // mkdir -p
+  for (size_t i = 1; i <= dir.size(); ++i)
+    if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+  const std::string tmp = path + ".tmp" + std::to_string(getpid()) + "_" +
+                          std::to_string(std::hash<std::thread::id>()(std::this_thread::get_id()));
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f) return;
+    f.write(code.data(), std::streamsize(code.size()));
+    if (!f) {
+      f.close();
+      std::remove(tmp.c_str());
+      return;
+    }
+  }
+  if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
 
 }  // namespace
+
+bool qtile_jit_available() { return rtc().ok; }
+
+void qtile_jit_cache_stats(int64_t* mem_hits, int64_t* disk_hits, int64_t* compiles) {
+  if (mem_hits) *mem_hits = g_mod_hits.load();
+  if (disk_hits) *disk_hits = g_disk_hits.load();
+  if (compiles) *compiles = g_compiles.load();
+}
 
 std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1) {
   std::ostringstream s;
@@ -573,13 +643,27 @@ int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
     for (size_t g = 0; g < groups; ++g) {
       src[g] = qtile_jit_source(qp, c128, cut[g], cut[g + 1]);
       auto it = g_cache.find(CacheKey{dev, src[g]});
-      if (it != g_cache.end()) mods[g] = it->second;
-      else todo.push_back(g);
+      if (it != g_cache.end()) {
+        mods[g] = it->second;
+        ++g_mod_hits;
+      } else {
+        todo.push_back(g);
+      }
     }
   }
+  const std::string dir = disk_cache_dir();
+  std::vector<std::string> path(groups);
+  std::vector<char> from_disk(groups, 0);
   std::vector<std::thread> th;
   for (size_t g : todo)
     th.emplace_back([&, g] {
+      if (!dir.empty()) {
+        path[g] = disk_cache_file(dir, src[g]);
+        if (disk_cache_read(path[g], code[g])) {
+          from_disk[g] = 1;
+          return;
+        }
+      }
       st[g] = qtile_jit_compile(src[g], code[g]);
       if (st[g] != QSV_OK) err[g] = get_error();
     });
@@ -590,7 +674,20 @@ int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     for (size_t g : todo) {
       auto m = std::make_shared<QJitModule>();
-      QSV_CHECK_HIP(hipModuleLoadData(&m->mod, code[g].data()));
+      if (from_disk[g] && hipModuleLoadData(&m->mod, code[g].data()) != hipSuccess) {
+        // unreadable cached object: compile it again and replace the file
+        (void)hipGetLastError();
+        m->mod = nullptr;
+        from_disk[g] = 0;
+        QSV_TRY(qtile_jit_compile(src[g], code[g]));
+      }
+      if (!from_disk[g]) {
+        QSV_CHECK_HIP(hipModuleLoadData(&m->mod, code[g].data()));
+        ++g_compiles;
+        if (!dir.empty()) disk_cache_write(dir, path[g], code[g]);
+      } else {
+        ++g_disk_hits;
+      }
       mods[g] = m;
       if (g_cache.size() >= kCacheMax) g_cache.clear();
       g_cache[CacheKey{dev, src[g]}] = m;

@@ -54,4 +54,11 @@ int qtile_jit_launch(const QJit& j, size_t i, void* psi, const QOp* ops, hipStre
 // Compile one source to a gfx950 code object (no device needed).
 int qtile_jit_compile(const std::string& src, std::vector<char>& code);
 
+// True when hiprtc loads, so compiled passes can be built.
+bool qtile_jit_available();
+
+// Module groups found in the process-wide cache, loaded from the on-disk
+// cache, and compiled, since the library was loaded.
+void qtile_jit_cache_stats(int64_t* mem_hits, int64_t* disk_hits, int64_t* compiles);
+
 }  // namespace qsv

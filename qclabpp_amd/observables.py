@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import QsvPauliTerm, check, lib
+from ._lib import QSV_AUTO, QsvPauliTerm, check, lib
 from .engine import _check_state, stream_handle
 from .gates import records
 
@@ -103,10 +103,11 @@ def apply_pauli_sum(psi_in, psi_out, H: PauliSum, stream=None) -> None:
                                         n, dt, arr, cnt, stream_handle(stream)))
 
 
-def adjoint_gradient(psi, lam, gates, H: PauliSum, theta=None, nparams=None, fuse_k: int = 5,
+def adjoint_gradient(psi, lam, gates, H: PauliSum, theta=None, nparams=None, fuse_k: int = QSV_AUTO,
                      stream=None):
     """(E, dE/dtheta) by the adjoint method; psi holds psi0 on entry and is
-    rewound to it; lam is scratch."""
+    rewound to it; lam is scratch.  fuse_k selects the forward pass's form
+    (default QSV_AUTO, the engine's fastest)."""
     n, dt = _check_state(psi)
     arr, cnt = records(gates)
     if nparams is None:

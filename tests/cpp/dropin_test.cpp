@@ -345,6 +345,54 @@ void circuit_checks(const char* tn) {
     CHECK(max_diff(res[0], res[2]) < 10 * tol<T>(), "%s plain vs tiled: %.3e", tn, max_diff(res[0], res[2]));
   }
 
+  // the default form from 22 qubits up is the compiled tiled passes; a
+  // second simulate of the same structure with other angles reuses the
+  // schedule and the compiled kernels (no hiprtc compile) and matches the
+  // fused form
+  {
+    const int nb = 22;
+    auto brick = [&](double phase) {
+      QCircuit<T> c(nb);
+      for (int layer = 0; layer < 6; ++layer) {
+        for (int qb = 0; qb < nb; ++qb) {
+          const real_t<T> th = real_t<T>(0.3 + 0.11 * qb + 0.7 * layer + phase);
+          if ((qb + layer) % 3 == 0) c.push_back(std::make_unique<RotationX<T>>(qb, th));
+          else if ((qb + layer) % 3 == 1) c.push_back(std::make_unique<RotationY<T>>(qb, th));
+          else c.push_back(std::make_unique<RotationZ<T>>(qb, th));
+        }
+        for (int qb = layer % 2; qb + 1 < nb; qb += 2) c.push_back(std::make_unique<CX<T>>(qb, qb + 1));
+      }
+      return c;
+    };
+    unsetenv("QCLAB_FUSE_K");
+    qsv_cache_stats s0{}, s1{}, s2{};
+    CHECK(qsv_cache_stats_get(&s0) == QSV_OK, "cache stats");
+    auto a = random_state<T>(nb, 41), b = a;
+    QCircuit<T> c1 = brick(0.0);
+    c1.simulate(a);
+    CHECK(qsv_cache_stats_get(&s1) == QSV_OK, "cache stats");
+    CHECK(s1.schedule_hits + s1.schedule_misses > s0.schedule_hits + s0.schedule_misses,
+          "%s default form at n=22 builds tiled passes", tn);
+    CHECK(s1.module_compiles + s1.module_hits + s1.module_disk_hits >
+              s0.module_compiles + s0.module_hits + s0.module_disk_hits,
+          "%s default form at n=22 runs compiled passes", tn);
+    QCircuit<T> c2 = brick(0.37);
+    auto a2 = a;
+    c2.simulate(a2);
+    CHECK(qsv_cache_stats_get(&s2) == QSV_OK, "cache stats");
+    CHECK(s2.schedule_hits == s1.schedule_hits + 1 && s2.schedule_misses == s1.schedule_misses,
+          "%s second simulate of one structure reuses the schedule (hits %lld -> %lld)", tn,
+          (long long)s1.schedule_hits, (long long)s2.schedule_hits);
+    CHECK(s2.module_compiles == s1.module_compiles && s2.module_hits > s1.module_hits,
+          "%s second simulate of one structure reuses the compiled passes", tn);
+    setenv("QCLAB_FUSE_K", "5", 1);
+    c1.simulate(b);
+    unsetenv("QCLAB_FUSE_K");
+    double scale = 0;
+    for (const T& x : b) scale = std::max(scale, double(std::abs(x)));
+    CHECK(max_diff(a, b) < tol<T>() * scale, "%s compiled passes vs fused at n=22: %.3e", tn, max_diff(a, b));
+  }
+
   // QASM round trip through the extended reader (no Phase: it is written as rz)
   QCircuit<T> q(5);
   q.push_back(std::make_unique<Hadamard<T>>(0));

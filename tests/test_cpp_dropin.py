@@ -44,10 +44,14 @@ def test_host_checks():
 
 
 @pytest.mark.gpu
-def test_dropin_on_gpu():
-    """Every gate class, circuits and device vectors on the GPU."""
+def test_dropin_on_gpu(tmp_path):
+    """Every gate class, circuits and device vectors on the GPU, and the
+    default compiled form at 22 qubits with its caches (compiled code
+    objects go to a fresh directory)."""
+    import os
     print(f"running {BIN}", flush=True)
-    res = subprocess.run([str(_binary())], capture_output=True, text=True, timeout=300)
+    env = dict(os.environ, QSV_JIT_CACHE_DIR=str(tmp_path))
+    res = subprocess.run([str(_binary())], capture_output=True, text=True, timeout=300, env=env)
     print(res.stdout[-4000:])
     assert res.returncode == 0, res.stdout[-4000:] + res.stderr[-2000:]
     assert "0 failed" in res.stdout
