@@ -74,7 +74,40 @@ def ref_lib() -> C.CDLL:
         _r.ref_circuit_matrix.argtypes = [C.c_int, C.c_int, gp, C.c_size_t, C.c_void_p]
         _r.ref_time_simulate_zero.argtypes = [C.c_int, C.c_int, gp, C.c_size_t,
                                               C.POINTER(C.c_double)]
+        _r.ref_qasm_gates.argtypes = [C.c_char_p, gp, C.c_size_t, C.POINTER(C.c_size_t),
+                                      C.POINTER(C.c_int)]
+        _r.ref_circuit_qasm.argtypes = [C.c_int, gp, C.c_size_t, C.c_char_p, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]
     return _r
+
+
+def ref_qasm_gates(path) -> tuple[int, list]:
+    """The reference's io::QASMFile on a file: (nbQubits, gates), each gate
+    a tuple (kind, q0, q1, ctrl_state, theta) as qasm_records.hpp makes it."""
+    from qclabpp_amd._lib import QsvGate
+    cap = 4096
+    arr = (QsvGate * cap)()
+    cnt = C.c_size_t()
+    nq = C.c_int()
+    st = ref_lib().ref_qasm_gates(str(path).encode(), arr, cap, C.byref(cnt), C.byref(nq))
+    if st:
+        raise RuntimeError(f"reference QASM reader status {st}")
+    out = [(g.kind, g.q[0], g.q[1], g.ctrl_state, g.theta) for g in arr[:min(cnt.value, cap)]]
+    return nq.value, out
+
+
+def ref_circuit_qasm(n: int, gates) -> str:
+    """The reference's QCircuit::toQASM text of a gate list (complex<double>)."""
+    arr, cnt = records(gates)
+    ln = C.c_size_t()
+    st = ref_lib().ref_circuit_qasm(n, arr, cnt, None, 0, C.byref(ln))
+    if st:
+        raise RuntimeError(f"reference QASM writer status {st}")
+    buf = C.create_string_buffer(ln.value + 1)
+    st = ref_lib().ref_circuit_qasm(n, arr, cnt, buf, ln.value + 1, C.byref(ln))
+    if st:
+        raise RuntimeError(f"reference QASM writer status {st}")
+    return buf.value.decode()
 
 
 def ref_time_simulate_zero(n: int, gates, dtype=np.complex128) -> float:

@@ -14,7 +14,12 @@
 #include <omp.h>
 #endif
 
+#include <cstdio>
+#include <sstream>
+#include <string>
+
 #include "qclab/QCircuit.hpp"
+#include "qclab/io/QASMFile.hpp"
 #include "qclab/qgates/CNOT.hpp"
 #include "qclab/qgates/CPhase.hpp"
 #include "qclab/qgates/CRotationX.hpp"
@@ -40,6 +45,7 @@
 #include "qclab/qgates/SWAP.hpp"
 #include "qclab/qgates/iSWAP.hpp"
 #include "qsv.h"
+#include "qasm_records.hpp"
 
 namespace {
 
@@ -263,6 +269,44 @@ int ref_time_simulate_zero(int n, int dtype, const qsv_gate* g, size_t ng, doubl
   if (dtype == QSV_C128) return run(std::complex<double>());
   if (dtype == QSV_C64) return run(std::complex<float>());
   return QSV_EDTYPE;
+}
+
+// io::QASMFile(filename).gates<QObject<complex<double>>>()
+// (include/qclab/io/QASMFile.hpp:42-152, src/io/QASMFile.cpp:16-53): the
+// gates the reference reads from a file, as records (qasm_records.hpp).
+// *count gets the number of gates (records beyond `cap` are dropped),
+// *nqubits the register size; "Failed parsing" goes to stderr as in the
+// reference.  Inputs on which the reference never returns (a last command
+// without ';', an empty command between two ';') must not be given.
+int ref_qasm_gates(const char* path, qsv_gate* out, size_t cap, size_t* count, int* nqubits) {
+  if (!path || !count || !nqubits) return QSV_EINVAL;
+  using T = std::complex<double>;
+  qclab::io::QASMFile f(path);
+  const auto gates = f.gates<qclab::QObject<T>>();
+  *nqubits = f.nbQubits();
+  *count = gates.size();
+  for (size_t i = 0; i < gates.size() && i < cap; ++i)
+    if (!qsv_oracle::qasm_record(*gates[i], out[i])) return QSV_EUNSUP;
+  return QSV_OK;
+}
+
+// QCircuit::toQASM (include/qclab/QCircuit.hpp:211-218 with the gates'
+// toQASM and include/qclab/qasm.hpp) of the records' circuit, complex<double>
+// (angles written "%.15f").  *len gets the text length; the text (NUL
+// terminated) is copied when it fits in `cap`.  Returns the writer's status
+// (nonzero: a gate without a QASM form).
+int ref_circuit_qasm(int n, const qsv_gate* g, size_t ng, char* out, size_t cap, size_t* len) {
+  if ((!g && ng) || !len) return QSV_EINVAL;
+  using T = std::complex<double>;
+  qclab::QCircuit<T> circ(n);
+  const int st = build_circuit(n, g, ng, circ);
+  if (st) return st;
+  std::ostringstream text;
+  const int w = circ.toQASM(text);
+  const std::string s = text.str();
+  *len = s.size();
+  if (out && cap > s.size()) std::memcpy(out, s.c_str(), s.size() + 1);
+  return w == 0 ? QSV_OK : QSV_EUNSUP;
 }
 
 int ref_max_threads(void) {

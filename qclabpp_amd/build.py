@@ -80,7 +80,7 @@ def build_cpp_tests(verbose: bool = True) -> Path:
     can be copied elsewhere and still run it.
     """
     CPP_TEST_BIN.parent.mkdir(parents=True, exist_ok=True)
-    cmd = [_hipcc(), "-std=c++17", "-O1", f"-I{ROOT / 'include'}",
+    cmd = [_hipcc(), "-std=c++17", "-O1", f"-I{ROOT / 'include'}", f"-I{ROOT / 'oracle'}",
            str(CPP_TESTS / "dropin_test.cpp"), "-o", str(CPP_TEST_BIN),
            f"-L{PKG}", "-lqclabpp_amd", "-Wl,-rpath,$ORIGIN/../../../qclabpp_amd"]
     res = subprocess.run(cmd, capture_output=True, text=True)

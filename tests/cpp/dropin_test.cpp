@@ -8,6 +8,12 @@
 //
 //   dropin_test          every check (needs a GPU)
 //   dropin_test --host   only the checks that do not touch the GPU
+//   dropin_test --qasm-dump FILE ref|ext
+//                        the gates io::QASMFile reads from FILE in the given
+//                        dialect, one record per line (no GPU)
+//   dropin_test --qasm-rewrite FILE
+//                        FILE read in the extended dialect and written back
+//                        with QCircuit::toQASM (no GPU)
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
@@ -22,6 +28,18 @@
 
 #include "qclab/QCircuit.hpp"
 #include "qclab/io/QASMFile.hpp"
+#include "qclab/qgates/CPhase.hpp"
+#include "qclab/qgates/CRotationX.hpp"
+#include "qclab/qgates/CRotationY.hpp"
+#include "qclab/qgates/CRotationZ.hpp"
+#include "qclab/qgates/Identity.hpp"
+#include "qclab/qgates/Phase45.hpp"
+#include "qclab/qgates/Phase90.hpp"
+#include "qclab/qgates/RotationXX.hpp"
+#include "qclab/qgates/RotationYY.hpp"
+#include "qclab/qgates/RotationZZ.hpp"
+#include "qclab/qgates/iSWAP.hpp"
+#include "qasm_records.hpp"
 #include "qclab/qgates/CNOT.hpp"
 #include "qclab/qgates/PointerGate1.hpp"
 #include "qclab/qgates/PointerGate2.hpp"
@@ -416,7 +434,9 @@ void circuit_checks(const char* tn) {
   auto a = random_state<T>(5, 8), b = a;
   q.simulate(a);
   q2.simulate(b);
-  CHECK(max_diff(a, b) < 1e-6, "%s QASM round trip state: %.3e", tn, max_diff(a, b));
+  // angles are written "%.15f" for double and "%.7f" for float
+  const double qtol = std::is_same_v<T, std::complex<double>> ? 1e-13 : 1e-5;
+  CHECK(max_diff(a, b) < qtol, "%s QASM round trip state: %.3e", tn, max_diff(a, b));
 }
 
 // checks with no GPU work: headers, QASM readers, matrices
@@ -478,7 +498,44 @@ void host_checks(const char* tn) {
   CHECK(!c1.canInsert(Hadamard<T>(2)) && c1.canInsert(Hadamard<T>(1)), "%s canInsert", tn);
 }
 
+// --qasm-dump: "nq N" then "kind q0 q1 ctrl_state theta" per gate
+int qasm_dump(const char* path, const char* dialect) {
+  using T = std::complex<double>;
+  const auto d = std::strcmp(dialect, "ext") == 0 ? io::QASMFile::Dialect::Extended
+                                                  : io::QASMFile::Dialect::Reference;
+  io::QASMFile f(path, d);
+  const auto gates = f.gates<QObject<T>>();
+  std::printf("nq %d\n", f.nbQubits());
+  for (const auto& g : gates) {
+    qsv_gate r;
+    if (!qsv_oracle::qasm_record(*g, r)) {
+      std::printf("unknown gate\n");
+      return 1;
+    }
+    std::printf("%d %d %d %d %.17g\n", r.kind, r.q[0], r.q[1], r.ctrl_state, r.theta);
+  }
+  return 0;
+}
+
+// --qasm-rewrite: read (extended dialect), write back with QCircuit::toQASM
+int qasm_rewrite(const char* path) {
+  using T = std::complex<double>;
+  io::QASMFile f(path, io::QASMFile::Dialect::Extended);
+  if (!f.error().empty()) {
+    std::fprintf(stderr, "%s\n", f.error().c_str());
+    return 1;
+  }
+  QCircuit<T> c(f.nbQubits());
+  for (auto& g : f.gates<QObject<T>>()) c.push_back(std::move(g));
+  std::ostringstream text;
+  if (c.toQASM(text) != 0) return 1;
+  std::fputs(text.str().c_str(), stdout);
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 3 && std::strcmp(argv[1], "--qasm-dump") == 0) return qasm_dump(argv[2], argv[3]);
+  if (argc > 2 && std::strcmp(argv[1], "--qasm-rewrite") == 0) return qasm_rewrite(argv[2]);
   const bool host_only = argc > 1 && std::strcmp(argv[1], "--host") == 0;
   host_checks<std::complex<double>>("c128");
   host_checks<std::complex<float>>("c64");
