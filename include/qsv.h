@@ -260,6 +260,15 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
                              size_t ngates, const double* theta, size_t nparams,
                              const qsv_pauli_term* terms, size_t nterms, int fuse_k,
                              double* energy, double* grad, void* stream);
+/* Build check of the compiled reverse sweep without a GPU: the inverse
+ * gates, last first, scheduled into two-state tiled passes with each
+ * parametrised gate's generator measured before it; every pass generated
+ * and compiled for gfx950 with hiprtc, not loaded.  *code_bytes,
+ * *npasses and *nrows (each optional) receive the code size, the pass count
+ * and the generator rows.  No counterpart in the reference; for build
+ * systems and CI hosts without a device.                                  */
+int qsv_adjoint_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
+                              size_t* code_bytes, int64_t* npasses, int64_t* nrows);
 
 /* ---- batched circuits (no counterpart in the reference) --------------------
  * Many independent circuits of uneven qubit counts, each started from

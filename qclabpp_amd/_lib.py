@@ -184,6 +184,8 @@ SIGNATURES = {
     "qsv_host_circuit_matrix": (_I, [_P, _I, _I, _GP, _SZ, _I]),
     "qsv_dev_nrmf": (_I, [_P, _P, _U64, _I, _DP, _P]),
     "qsv_circuit_nrmf": (_I, [_I, _I, _GP, _SZ, _GP, _SZ, _I, _DP]),
+    "qsv_adjoint_compile_check": (_I, [_I, _I, _GP, _SZ, C.POINTER(_SZ), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64)]),
     "qsv_dev_adjoint_gradient": (_I, [_P, _P, _I, _I, _GP, _SZ, _DP, _SZ, C.POINTER(QsvPauliTerm),
                                       _SZ, _I, _DP, _DP, _P]),
 }

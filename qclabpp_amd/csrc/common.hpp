@@ -132,6 +132,13 @@ struct KOp {
   // adjoint support: Pauli generator of a rotation, -1 if none
   int param = -1;
   int gen = -1;        // gate kind of the generator owner (QSV_GATE_*)
+  // Adjoint reverse sweep in compiled passes: when grow >= 0, the op's
+  // generator is measured as result row grow right before the op:
+  // G|y> = gph * (-1)^popc(y & gz) * [(y & gpm) == gpv] |y ^ gflip> over
+  // address bits, and the row is gw_re * Re<lam|G|phi> + gw_im * Im<lam|G|phi>.
+  int grow = -1;
+  uint64_t gflip = 0, gz = 0, gpm = 0, gpv = 0;
+  double gph_re = 1, gph_im = 0, gw_re = 0, gw_im = 0;
 };
 
 // Lower one gate record to a kernel op; validates qubits against n.

@@ -19,8 +19,10 @@
 // diag(1, e^{i theta}) give dE/dtheta = -2 Im<lam|Pi_1|phi> (Pi_1 projects the
 // target, and the control, onto the phased state).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -28,6 +30,8 @@
 #include "observables.hpp"
 #include "plain.hpp"
 #include "program.hpp"
+#include "qtile.hpp"
+#include "qtile_jit.hpp"
 #include "tile.hpp"
 
 namespace qsv {
@@ -142,6 +146,105 @@ bool generator(const qsv_gate& g, int n, PauliOp& G, double& wr, double& wi) {
       return false;
   }
   return true;
+}
+
+static_assert(kAdjPartStride == kRedBlocks, "compiled adjoint passes write reduction-row partials");
+
+// Tile bits of the compiled reverse sweep: one shared LDS tile of 2^12
+// amplitudes (64 KiB for complex128) that both states pass through.
+constexpr int kAdjJitTileBits = 12;
+
+// Which reverse sweep runs: compiled two-state passes where hiprtc loads
+// and the state holds a tile (QSV_ADJOINT=interp forces the interpreted
+// LDS passes, =jit requires the compiled ones).
+int adjoint_form(int n) {
+  const char* e = std::getenv("QSV_ADJOINT");
+  const std::string f = e ? e : "auto";
+  if (f == "interp") return 0;
+  const bool ok = n >= kAdjJitTileBits && qtile_jit_available();
+  if (f == "jit" && !ok) return -1;
+  return ok ? 1 : 0;
+}
+
+// The reverse sweep as compiled passes over both states (the inverse gates,
+// last first, scheduled as tiled passes; each parametrised gate's generator
+// is measured just before its inverse, see Q_GEN).  Appends one result row
+// per parametrised gate: its parameter, and the scale |lam|^2 the pass's
+// scaled rotations took out of both states before it.  Row partials go to
+// partials[(row0 + r) * kRedBlocks + block].
+int adjoint_schedule(int n, int dtype, const std::vector<qsv_gate>& gs, QProgram& qp,
+                     std::vector<int>& pass_row0, std::vector<int>& row_param,
+                     std::vector<double>& row_scale) {
+  std::vector<KOp> rev;
+  rev.reserve(gs.size());
+  for (size_t i = gs.size(); i-- > 0;) {
+    qsv_gate c = gs[i];
+    c.op = QSV_OP_C;
+    KOp o;
+    QSV_TRY(lower_gate(c, n, o));
+    if (gs[i].param >= 0) {
+      PauliOp G;
+      double wr = 0, wi = 0;
+      if (!generator(gs[i], n, G, wr, wi))
+        return fail(QSV_EUNSUP, "gate kind " + std::to_string(gs[i].kind) + " is not differentiable");
+      o.grow = int(i);
+      o.gflip = G.flip;
+      o.gz = G.zmask;
+      o.gpm = G.pmask;
+      o.gpv = G.pval;
+      o.gph_re = G.phase_re;
+      o.gph_im = G.phase_im;
+      o.gw_re = wr;
+      o.gw_im = wi;
+    }
+    rev.push_back(o);
+  }
+  QSV_TRY(compile_qtile(rev, n, dtype == QSV_C128 ? 3 : 4, qp, kAdjJitTileBits, true));
+  qtile_scale_rotations(qp);
+  // rows numbered pass by pass, so each pass writes a contiguous block
+  pass_row0.assign(qp.passes.size(), 0);
+  int rows = 0;
+  for (size_t i = 0; i < qp.passes.size(); ++i) {
+    const QPass& P = qp.passes[i];
+    pass_row0[i] = rows;
+    int local = 0;
+    for (int o = qp.phases[P.phase_begin].op_begin; o < qp.phases[P.phase_end - 1].op_end; ++o) {
+      QOp& x = qp.ops[o];
+      if (x.kind != Q_GEN) continue;
+      row_param.push_back(gs[size_t(x.a)].param);
+      row_scale.push_back(x.m[0]);
+      x.a = local++;
+    }
+    rows += local;
+  }
+  return QSV_OK;
+}
+
+int adjoint_reverse_jit(void* psi, void* lam, int n, int dtype, const std::vector<qsv_gate>& gs,
+                        double2* partials, int row0, std::vector<int>& row_param,
+                        std::vector<double>& row_scale, hipStream_t st, bool& done) {
+  done = false;
+  QProgram qp;
+  std::vector<int> pass_row0;
+  QSV_TRY(adjoint_schedule(n, dtype, gs, qp, pass_row0, row_param, row_scale));
+  if (qtile_jit_adjoint_check(qp) != QSV_OK) {
+    // a pass with more generator rows than LDS holds: the caller falls back
+    row_param.clear();
+    row_scale.clear();
+    return QSV_OK;
+  }
+  QJit qj;
+  QSV_TRY(qtile_jit_build_adjoint(qp, dtype == QSV_C128, qj));
+  DevArray<QOp> d_ops;
+  QSV_TRY(d_ops.upload(qp.ops));
+  for (size_t i = 0; i < qp.passes.size(); ++i) {
+    double* part = reinterpret_cast<double*>(partials + size_t(row0 + pass_row0[i]) * kRedBlocks);
+    QSV_TRY(qtile_jit_launch_adjoint(qj, i, psi, lam, d_ops.p, part, st));
+  }
+  // the op table is freed with d_ops: finish before returning
+  QSV_CHECK_HIP(hipStreamSynchronize(st));
+  done = true;
+  return QSV_OK;
 }
 
 template <typename R>
@@ -259,6 +362,31 @@ int qsv_dev_apply_pauli_sum(const void* in, void* out, int n, int dtype,
   return QSV_OK;
 }
 
+int qsv_adjoint_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
+                              size_t* code_bytes, int64_t* npasses, int64_t* nrows) {
+  if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
+  if (n < kAdjJitTileBits || n > 40) return fail(QSV_EINVAL, "compiled adjoint passes need n in [12, 40]");
+  if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
+  std::vector<qsv_gate> gs(gates, gates + ngates);
+  for (const qsv_gate& g : gs)
+    if (g.param >= 0 && g.op != QSV_OP_N) return fail(QSV_EUNSUP, "parametrised gates must use Op 'N'");
+  QProgram qp;
+  std::vector<int> pass_row0, rp;
+  std::vector<double> rs;
+  QSV_TRY(adjoint_schedule(n, dtype, gs, qp, pass_row0, rp, rs));
+  QSV_TRY(qtile_jit_adjoint_check(qp));
+  size_t total = 0;
+  for (size_t i = 0; i < qp.passes.size(); ++i) {
+    std::vector<char> code;
+    QSV_TRY(qtile_jit_compile(qtile_jit_adjoint_source(qp, dtype == QSV_C128, i, i + 1), code));
+    total += code.size();
+  }
+  if (code_bytes) *code_bytes = total;
+  if (npasses) *npasses = int64_t(qp.passes.size());
+  if (nrows) *nrows = int64_t(rp.size());
+  return QSV_OK;
+}
+
 int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_gate* gates,
                              size_t ngates, const double* theta, size_t nparams,
                              const qsv_pauli_term* terms, size_t nterms, int fuse_k,
@@ -306,6 +434,25 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
   auto* red = partials + nrows * kRedBlocks;
   // energy row: <phi|lam>
   QSV_TRY(inner_any(dtype, psi, lam, n, PauliOp(), partials, st));
+  const int form = adjoint_form(n);
+  if (form < 0) return fail(QSV_EUNSUP, "QSV_ADJOINT=jit: compiled passes need hiprtc and n >= 12");
+  if (form == 1) {
+    std::vector<int> rp;
+    std::vector<double> rs;
+    bool done = false;
+    QSV_TRY(adjoint_reverse_jit(psi, lam, n, dtype, gs, partials, 1, rp, rs, st, done));
+    if (done) {
+      const int rows = 1 + int(rp.size());
+      QSV_TRY(launch_reduce_rows(partials, rows, red, st));
+      std::vector<double2> host(rows);
+      QSV_CHECK_HIP(hipMemcpyAsync(host.data(), red, rows * sizeof(double2), hipMemcpyDeviceToHost, st));
+      QSV_CHECK_HIP(hipStreamSynchronize(st));
+      if (energy) *energy = host[0].x;
+      for (size_t p = 0; p < nparams; ++p) grad[p] = 0;
+      for (int r = 1; r < rows; ++r) grad[rp[r - 1]] += rs[r - 1] * host[r].x;
+      return QSV_OK;
+    }
+  }
   int rows = 1;
   std::vector<int> row_param(1, -1);
   std::vector<double> row_wr(1, 0), row_wi(1, 0);

@@ -43,7 +43,9 @@ std::vector<KOp> merge_one_qubit_runs(const std::vector<KOp>& in, int n) {
   std::vector<int> last(n, -1);
   for (const KOp& op : in) {
     if (op.kind == K_NOP) continue;
-    if (is_one_qubit(op) && last[op.q[0]] >= 0) {
+    // an op whose generator is measured before it keeps its place (the ops
+    // before it must have acted); later ops may still merge into it
+    if (is_one_qubit(op) && last[op.q[0]] >= 0 && op.grow < 0) {
       KOp& prev = ops[last[op.q[0]]];
       cd a[4], b[4];
       as2x2(op, a);
@@ -531,9 +533,52 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
           }
         }
       };
+      // the generator of op as a Q_GEN over the registers: for register p
+      // (logical index y = lg[p]) k_p = c(y) * (gw_re - i gw_im) with c(y)
+      // the generator's coefficient of |y ^ flip><y|, and the partner of p
+      // is the register of y ^ flip, p ^ d
+      auto emit_gen = [&](const KOp& op) {
+        int flipL = 0, zL = 0, pmL = 0, pvL = 0;
+        uint64_t seen = 0;
+        for (int j = 0; j < op.nq; ++j) {
+          const int ab = n - 1 - op.q[j], r = rb(op.q[j]);
+          seen |= uint64_t(1) << ab;
+          flipL |= int(op.gflip >> ab & 1) << r;
+          zL |= int(op.gz >> ab & 1) << r;
+          pmL |= int(op.gpm >> ab & 1) << r;
+          pvL |= int(op.gpv >> ab & 1) << r;
+        }
+        if ((op.gflip | op.gz | op.gpm | op.gpv) & ~seen)
+          return fail(QSV_EINVAL, "adjoint: generator acts outside its gate's qubits");
+        const int D = fr.phys[flipL] ^ fr.phys[0];
+        const cd ph(op.gph_re, op.gph_im), w(op.gw_re, -op.gw_im);
+        QOp o{};
+        o.kind = Q_GEN;
+        o.a = op.grow;
+        o.d = D;
+        bool commutes = true;
+        for (int p = 0; p < NR; ++p) {
+          const int y = fr.lg[p];
+          if ((y & pmL) != pvL) continue;
+          const cd k = ph * w * (__builtin_popcount(y & zL) & 1 ? -1.0 : 1.0);
+          o.mf[2 * p] = float(k.real());
+          o.mf[2 * p + 1] = float(k.imag());
+          if (o.mf[2 * p] != k.real() || o.mf[2 * p + 1] != k.imag())
+            return fail(QSV_EINVAL, "adjoint: generator weight is not a small constant");
+          o.act |= 1u << p;
+          if (D && diag[p] != diag[p ^ D]) commutes = false;
+        }
+        // pending factors of unit modulus that agree on both members cancel
+        // in conj(lam) * phi; otherwise apply them first
+        if (!commutes) flush_diag();
+        o.m[0] = 1.0;
+        out.ops.push_back(o);
+        return QSV_OK;
+      };
       open_phase();
       for (int oi : g.ops) {
         const KOp& op = mine[oi];
+        if (op.grow >= 0) QSV_TRY(emit_gen(op));
         const int t0 = rb(op.q[0]);
         const int t1 = op.nq > 1 ? rb(op.q[1]) : -1;
         auto bit = [](int i, int r) { return (i >> r) & 1; };
@@ -680,6 +725,7 @@ double qtile_fp_lane_ops(const QProgram& qp) {
       case Q_CPAIR: per_thread += 16 * __builtin_popcount(o.act); break;
       case Q_DIAGR: per_thread += 4 * __builtin_popcount(uint32_t(o.b)); break;
       case Q_DENSE2: per_thread += 64 * (NR / 4); break;
+      case Q_GEN: per_thread += 3 * __builtin_popcount(o.act); break;
       default: break;
     }
   }
@@ -735,6 +781,9 @@ void qtile_scale_rotations(QProgram& qp) {
           set_c(x, 1, first ? 1.0 : 0.0);
           lam *= f;
           ++nscaled;
+        } else if (x.kind == Q_GEN) {
+          // both states are stored divided by lam here
+          x.m[0] = std::norm(lam);
         } else if (x.kind == Q_DIAGR) {
           // take out the most common entry (unit entries of registers outside b included)
           cd e[NR];
@@ -804,7 +853,7 @@ std::string schedule_key(const std::vector<KOp>& in, int n, int line_bits, int t
   put(tile_bits);
   put(relabel);
   for (const KOp& op : in) {
-    put(op.kind | op.nq << 8 | (op.cs & 1) << 12);
+    put(op.kind | op.nq << 8 | (op.cs & 1) << 12 | (op.grow >= 0) << 13);
     for (int i = 0; i < op.nq; ++i) k.push_back(char(op.q[i]));
   }
   return k;

@@ -45,6 +45,12 @@ enum QKind : int32_t {
   // k = re(m0); re(m1) = 1 selects the first form, 0 the second.
   Q_ROTS = 10,    // [[1, -k], [k, 1]] or [[k, -1], [1, k]]
   Q_XLS = 11,     // [[1, ik], [ik, 1]] or [[k, i], [i, k]]
+  // Adjoint reverse sweep only (two states, phi in v and lam in w): result
+  // row `a` (pass-local) gets sum over registers p in `act` of
+  // Re(k_p * conj(w[p ^ d]) * v[p]), k_p = (mf[2p], mf[2p+1]) -- small
+  // integer constants of the circuit's structure -- and m[0] holds the
+  // factor |lam|^2 the scaled rotations took out of both states before it.
+  Q_GEN = 12,
 };
 
 struct QOp {

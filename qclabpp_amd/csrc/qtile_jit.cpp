@@ -125,7 +125,7 @@ __device__ __forceinline__ void u_d2(C& a0, C& a1, C& a2, C& a3, const QC R* cf,
 }
 )QSV";
 
-void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128) {
+void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128, const char* arr = "v") {
   // index of coefficient slot 0 of op k in the QOp array read as R[]
   const size_t stride = c128 ? sizeof(QOp) / sizeof(double) : sizeof(QOp) / sizeof(float);
   const size_t base = k * stride + (c128 ? offsetof(QOp, m) / sizeof(double) : offsetof(QOp, mf) / sizeof(float));
@@ -135,7 +135,7 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128) {
     for (int p = 0; p < NR; ++p)
       if (!(p & K)) body(p, p ^ o.d, bool(o.roles >> p & 1), bool(o.act >> p & 1));
   };
-  auto v = [](int p) { return "v[" + std::to_string(p) + "]"; };
+  auto v = [arr](int p) { return std::string(arr) + "[" + std::to_string(p) + "]"; };
   s << "    {\n";
   switch (o.kind) {
     case Q_DIAGR:
@@ -448,6 +448,290 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
     << "}\n";
 }
 
+// ---- adjoint reverse sweep: passes over two states -------------------------
+//
+// The same tile walk and phases as emit_pass, over phi (psi, registers v)
+// and lam (registers w) at once: every op is applied to both, and a Q_GEN
+// op adds Re(k_p conj(w[p ^ d]) v[p]) over the registers into its row.
+// Both states share one LDS tile: a phase change moves phi through it, then
+// lam, so the block keeps the forward passes' LDS footprint (two blocks of
+// 64 KiB per CU for complex128).  Row sums: each thread's FP64 sum is
+// reduced over the 16 lanes of a DPP row (4 steps), and the row's first
+// lane adds it to its own LDS slot (4 slots per wave), or, for passes with
+// more than 128 rows, over the wave (readlane) into one slot per wave.  At
+// the end each block adds its slots in a fixed order and writes one FP64
+// partial per row and block, so the gradient does not depend on timing.
+
+const char* kAdjPrelude = R"QSV(
+__device__ __forceinline__ long long qta_bits(double x) { return __builtin_bit_cast(long long, x); }
+__device__ __forceinline__ double qta_dbl(long long b) { return __builtin_bit_cast(double, b); }
+template <int CTRL>
+__device__ __forceinline__ double qta_dpp(double x) {
+  const long long b = qta_bits(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xf, 0xf, false);
+  return qta_dbl((long long)(unsigned)lo | ((long long)hi << 32));
+}
+// sum over the 16 lanes of each DPP row: quad_perm [1,0,3,2], [2,3,0,1],
+// then row_ror 4 and 8; every lane of the row ends with the row's sum
+__device__ __forceinline__ double qta_rowsum(double t) {
+  t += qta_dpp<0xB1>(t);
+  t += qta_dpp<0x4E>(t);
+  t += qta_dpp<0x124>(t);
+  t += qta_dpp<0x128>(t);
+  return t;
+}
+__device__ __forceinline__ double qta_lane(double t, int lane) {
+  const long long b = qta_bits(t);
+  const int lo = __builtin_amdgcn_readlane(int(b), lane);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+  return qta_dbl((long long)(unsigned)lo | ((long long)hi << 32));
+}
+// 4 slots per wave: slot tid >> 4 (the DPP row)
+__device__ __forceinline__ void qta_acc4(double t, double* row, unsigned tid) {
+  t = qta_rowsum(t);
+  if ((tid & 15u) == 0u) row[tid >> 4] += t;
+}
+// 1 slot per wave: the four row sums added in lane order
+__device__ __forceinline__ void qta_acc1(double t, double* row, unsigned tid) {
+  t = qta_rowsum(t);
+  const double s = ((qta_lane(t, 0) + qta_lane(t, 16)) + qta_lane(t, 32)) + qta_lane(t, 48);
+  if ((tid & 63u) == 0u) row[tid >> 6] += s;
+}
+)QSV";
+
+// Rows a pass's Q_GEN ops write (pass-local indices 0..rows-1).
+int adj_pass_rows(const QProgram& qp, size_t pi) {
+  const QPass& P = qp.passes[pi];
+  int rows = 0;
+  for (int o = qp.phases[P.phase_begin].op_begin; o < qp.phases[P.phase_end - 1].op_end; ++o)
+    if (qp.ops[o].kind == Q_GEN) rows = std::max(rows, qp.ops[o].a + 1);
+  return rows;
+}
+
+void emit_gen(std::ostringstream& s, const QOp& o, bool c128, int slots_per_wave) {
+  const char* cv = c128 ? "" : "(double)";
+  s << "    {  // generator row " << o.a << "\n"
+    << "      double t = 0.0;\n";
+  for (int p = 0; p < NR; ++p) {
+    if (!(o.act >> p & 1u)) continue;
+    const int q = p ^ o.d;
+    const double kr = o.mf[2 * p], ki = o.mf[2 * p + 1];
+    const std::string l = "w[" + std::to_string(q) + "]", f = "v[" + std::to_string(p) + "]";
+    // Re(k conj(l) f) = kr (l.x f.x + l.y f.y) - ki (l.x f.y - l.y f.x)
+    if (kr != 0)
+      s << "      t = fma(" << std::to_string(kr) << ", " << cv << l << ".x * " << cv << f << ".x + " << cv
+        << l << ".y * " << cv << f << ".y, t);\n";
+    if (ki != 0)
+      s << "      t = fma(" << std::to_string(-ki) << ", " << cv << l << ".x * " << cv << f << ".y - " << cv
+        << l << ".y * " << cv << f << ".x, t);\n";
+  }
+  s << "      qta_acc" << slots_per_wave << "(t, acc + " << o.a << " * NSLOT, tid);\n"
+    << "    }\n";
+}
+
+void emit_adj_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) {
+  const QPass& P = qp.passes[pi];
+  const int T = qp.tile_bits;
+  const unsigned threads = 1u << (T - kQRegBits);
+  const int K = P.phase_end - P.phase_begin;
+  const QPhase& F = qp.phases[P.phase_begin];
+  const QPhase& L = qp.phases[P.phase_end - 1];
+  const int rows = adj_pass_rows(qp, pi);
+  const int spw = rows <= 128 ? 4 : 1;  // LDS slots per wave and row
+  const unsigned nslot = threads / 64 * spw;
+  const uint32_t line_in = line_mask(P.tbits, T, c128), line_out = line_mask(P.tbits_out, T, c128);
+  const bool din = !(F.regmask & line_in);
+  const bool relabels = std::memcmp(P.tbits, P.tbits_out, sizeof(P.tbits)) != 0;
+  bool dout = !(L.regmask & line_out) && !(K == 1 && din && relabels);
+  std::vector<std::vector<int>> ords(K);
+  for (int k = 0; k < K; ++k) {
+    const QPhase& Q = qp.phases[P.phase_begin + k];
+    ords[k] = lane_order(T, Q.regmask, k == 0 && din ? line_in : 0u, k == K - 1 && dout ? line_out : 0u);
+  }
+  if (dout)
+    for (int j = kLaneBitsJit; j < int(ords[K - 1].size()); ++j)
+      if (line_out >> ords[K - 1][j] & 1u) dout = false;
+  if (!dout) ords[K - 1] = lane_order(T, L.regmask, K == 1 && din ? line_in : 0u, 0u);
+  const uint32_t tile_order = ((1u << kQRegBits) - 1) << (T - kQRegBits);
+  // registers: two states of 16 amplitudes; complex128 keeps two waves per SIMD
+  const int wpe = c128 ? 2 : 4;
+  s << "#define NSLOT " << nslot << "u\n"
+    << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(" << wpe
+    << ")))\n"
+    << "void qta_" << pi << "(C* __restrict__ psi, C* __restrict__ lam, const QC R* cf0, unsigned long long ntiles, "
+    << "double* __restrict__ part) {\n";
+  const bool lds = !(din && dout && K == 1);
+  if (lds) s << "  __shared__ C tile[" << (1u << T) << "];\n";
+  if (rows) s << "  __shared__ double acc[" << rows * nslot << "];\n";
+  s << "  const unsigned tid = threadIdx.x;\n";
+  if (rows)
+    s << "  for (unsigned i = tid; i < " << rows * nslot << "u; i += " << threads << "u) acc[i] = 0.0;\n"
+      << "  __syncthreads();\n";
+  s << "  auto tile_base = [&](unsigned long long t) {\n"
+    << "    unsigned long long b = 0;\n";
+  for (int r = 0, sh = 0; r < P.nruns; ++r) {
+    s << "    b |= ((t >> " << sh << ") & " << u64((uint64_t(1) << P.rlen[r]) - 1) << ") << " << int(P.rdst[r])
+      << ";\n";
+    sh += P.rlen[r];
+  }
+  s << "    return b;\n"
+    << "  };\n";
+  std::vector<uint64_t> in_ofs(NR), out_ofs(NR);
+  for (int i = 0; i < NR; ++i) {
+    in_ofs[i] = din ? gofs(P.tbits, T, qtile_swizzle(F.regpos[i])) : gofs(P.tbits, T, threads * uint32_t(i));
+    out_ofs[i] = dout ? gofs(P.tbits_out, T, qtile_swizzle(L.regpos_out[i]))
+                      : gofs(P.tbits_out, T, threads * uint32_t(i));
+  }
+  const std::vector<int> ord_tile = lane_order(T, tile_order, 0u, 0u);
+  s << "  const unsigned long long gin = " << thread_gofs(P.tbits, din ? ords[0] : ord_tile) << ";\n"
+    << "  const unsigned long long gout = " << thread_gofs(P.tbits_out, dout ? ords[K - 1] : ord_tile)
+    << ";\n";
+  auto arr = [&](const char* name, const std::vector<uint64_t>& v) {
+    s << "  const unsigned long long " << name << "[16] = {";
+    for (int i = 0; i < NR; ++i) s << (i ? ", " : "") << u64(v[i]);
+    s << "};\n";
+  };
+  arr("oin", in_ofs);
+  arr("oout", out_ofs);
+  s << "  const unsigned tslot = swz(tid);\n"
+    << "  const unsigned long long per = (ntiles + gridDim.x - 1) / gridDim.x;\n"
+    << "  unsigned long long t = blockIdx.x * per;\n"
+    << "  const unsigned long long tend = t + per < ntiles ? t + per : ntiles;\n"
+    << "  for (; t < tend; ++t) {\n"
+    << "    const unsigned long long base = tile_base(t);\n"
+    << "    C v[16], w[16];\n";
+  // the thread's tile-local position expression of phase k
+  auto position = [&](int k) {
+    std::ostringstream e;
+    e << "      unsigned y = tid;\n"
+      << "      asm volatile(\"\" : \"+v\"(y));\n"
+      << "      const unsigned x = 0u";
+    for (size_t j = 0; j < ords[k].size(); ++j) e << " | (((y >> " << j << ") & 1u) << " << ords[k][j] << ")";
+    e << ";\n"
+      << "      const unsigned tp = swz(x);\n";
+    return e.str();
+  };
+  auto gather = [&](const char* a, int k) {
+    const QPhase& Q = qp.phases[P.phase_begin + k];
+    for (int i = 0; i < NR; ++i) s << "      " << a << "[" << i << "] = tile[tp ^ " << Q.regpos[i] << "u];\n";
+  };
+  auto scatter = [&](const char* a, int k) {
+    const QPhase& Q = qp.phases[P.phase_begin + k];
+    for (int i = 0; i < NR; ++i) s << "      tile[tp ^ " << Q.regpos_out[i] << "u] = " << a << "[" << i << "];\n";
+  };
+  // tile start
+  if (din) {
+    s << "    {\n"
+      << "      const unsigned long long b0 = base + gin;\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) v[j] = ldnt(&psi[b0 + oin[j]]);\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) w[j] = ldnt(&lam[b0 + oin[j]]);\n"
+      << "    }\n";
+  } else {
+    // both states in tile order, then each through LDS into phase 0's registers
+    s << "    {\n"
+      << "      const unsigned long long b0 = base + gin;\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) v[j] = ldnt(&psi[b0 + oin[j]]);\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) w[j] = ldnt(&lam[b0 + oin[j]]);\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) tile[tslot ^ swz(" << threads << "u * j)] = v[j];\n"
+      << "      __syncthreads();\n"
+      << position(0);
+    gather("v", 0);
+    s << "      __syncthreads();\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) tile[tslot ^ swz(" << threads << "u * j)] = w[j];\n"
+      << "      __syncthreads();\n";
+    gather("w", 0);
+    s << "    }\n";
+  }
+  for (int k = 0; k < K; ++k) {
+    const int ph = P.phase_begin + k;
+    const QPhase& Q = qp.phases[ph];
+    s << "    {  // phase " << ph << "\n"
+      << "      const QC R* cf = cf0;\n"
+      << "      asm volatile(\"\" : \"+s\"(cf));\n";
+    for (int o = Q.op_begin; o < Q.op_end; ++o) {
+      if (qp.ops[o].kind == Q_GEN) {
+        emit_gen(s, qp.ops[o], c128, spw);
+      } else {
+        emit_op(s, qp.ops[o], size_t(o), c128, "v");
+        emit_op(s, qp.ops[o], size_t(o), c128, "w");
+      }
+    }
+    s << "    }\n";
+    if (k + 1 < K) {
+      // phase change: phi, then lam, through the shared tile
+      s << "    {\n" << position(k);
+      scatter("v", k);
+      s << "    }\n"
+        << "    __syncthreads();\n"
+        << "    {\n" << position(k + 1);
+      gather("v", k + 1);
+      s << "    }\n"
+        << "    __syncthreads();\n"
+        << "    {\n" << position(k);
+      scatter("w", k);
+      s << "    }\n"
+        << "    __syncthreads();\n"
+        << "    {\n" << position(k + 1);
+      gather("w", k + 1);
+      s << "    }\n";
+      // the last phase stores straight to HBM: the next tile's first LDS
+      // writes must wait until every thread has gathered
+      if (k + 1 == K - 1 && dout) s << "    __syncthreads();\n";
+    }
+  }
+  if (dout) {
+    if (K == 1 && !din) s << "    __syncthreads();\n";
+    s << "    {\n"
+      << "      const unsigned long long bo = base + gout;\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], v[j]);\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) stnt(&lam[bo + oout[j]], w[j]);\n"
+      << "    }\n";
+  } else {
+    s << "    {\n" << position(K - 1);
+    scatter("v", K - 1);
+    s << "    }\n"
+      << "    __syncthreads();\n"
+      << "    {\n"
+      << "      const unsigned long long bo = base + gout;\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], tile[tslot ^ swz(" << threads << "u * j)]);\n"
+      << "    }\n"
+      << "    __syncthreads();\n"
+      << "    {\n" << position(K - 1);
+    scatter("w", K - 1);
+    s << "    }\n"
+      << "    __syncthreads();\n"
+      << "    {\n"
+      << "      const unsigned long long bo = base + gout;\n"
+      << "#pragma unroll\n"
+      << "      for (int j = 0; j < 16; ++j) stnt(&lam[bo + oout[j]], tile[tslot ^ swz(" << threads << "u * j)]);\n"
+      << "    }\n"
+      << "    __syncthreads();\n";
+  }
+  s << "  }\n";
+  if (rows) {
+    s << "  __syncthreads();\n"
+      << "  for (unsigned r = tid; r < " << rows << "u; r += " << threads << "u) {\n"
+      << "    double sum = 0.0;\n"
+      << "    for (unsigned k = 0; k < NSLOT; ++k) sum += acc[r * NSLOT + k];\n"
+      << "    const unsigned long long at = 2ull * ((unsigned long long)r * " << kAdjPartStride << "ull + blockIdx.x);\n"
+      << "    part[at] = sum;\n"
+      << "    part[at + 1] = 0.0;\n"
+      << "  }\n";
+  }
+  s << "}\n"
+    << "#undef NSLOT\n";
+}
+
 // ---- hiprtc, loaded on first use ----------------------------------------
 struct Rtc {
   bool ok = false;
@@ -585,6 +869,11 @@ std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1
 int qtile_jit_compile(const std::string& src, std::vector<char>& code) {
   const Rtc& r = rtc();
   if (!r.ok) return fail(QSV_EUNSUP, "specialised tiled passes need hiprtc: " + r.why);
+  // QSV_JIT_DUMP_DIR: keep every generated source there, for inspection
+  if (const char* d = std::getenv("QSV_JIT_DUMP_DIR"); d && *d) {
+    std::ofstream f(disk_cache_file(d, src).substr(0, disk_cache_file(d, src).size() - 3) + ".hip");
+    f << src;
+  }
   hiprtcProgram prog = nullptr;
   if (r.create(&prog, src.c_str(), "qtile_pass.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
     return fail(QSV_EHIP, "hiprtcCreateProgram failed");
@@ -610,7 +899,12 @@ int qtile_jit_compile(const std::string& src, std::vector<char>& code) {
   return n ? QSV_OK : fail(QSV_EHIP, "hiprtc returned an empty code object");
 }
 
-int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
+namespace {
+
+// Generate (source of passes [p0, p1)), compile or take from the caches,
+// and load the kernels named prefix + pass index.
+template <typename Gen>
+int build_modules(const QProgram& qp, const char* prefix, Gen gen, QJit& out) {
   out = QJit();
   const size_t np = qp.passes.size();
   if (!np) return QSV_OK;
@@ -641,7 +935,7 @@ int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     for (size_t g = 0; g < groups; ++g) {
-      src[g] = qtile_jit_source(qp, c128, cut[g], cut[g + 1]);
+      src[g] = gen(cut[g], cut[g + 1]);
       auto it = g_cache.find(CacheKey{dev, src[g]});
       if (it != g_cache.end()) {
         mods[g] = it->second;
@@ -699,10 +993,51 @@ int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
   out.threads = 1u << (qp.tile_bits - kQRegBits);
   for (size_t g = 0; g < groups; ++g)
     for (size_t i = cut[g]; i < cut[g + 1]; ++i) {
-      const std::string name = "qtp_" + std::to_string(i);
+      const std::string name = prefix + std::to_string(i);
       QSV_CHECK_HIP(hipModuleGetFunction(&out.fn[i], mods[g]->mod, name.c_str()));
       out.ntiles[i] = qp.passes[i].ntiles;
     }
+  return QSV_OK;
+}
+
+}  // namespace
+
+int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
+  return build_modules(qp, "qtp_", [&](size_t p0, size_t p1) { return qtile_jit_source(qp, c128, p0, p1); },
+                       out);
+}
+
+std::string qtile_jit_adjoint_source(const QProgram& qp, bool c128, size_t p0, size_t p1) {
+  std::ostringstream s;
+  s << "#define REAL " << (c128 ? "double" : "float") << "\n" << kPrelude << kAdjPrelude;
+  for (size_t i = p0; i < p1 && i < qp.passes.size(); ++i) emit_adj_pass(s, qp, i, c128);
+  return s.str();
+}
+
+int qtile_jit_adjoint_check(const QProgram& qp) {
+  for (size_t i = 0; i < qp.passes.size(); ++i) {
+    const int rows = adj_pass_rows(qp, i);
+    const unsigned threads = 1u << (qp.tile_bits - kQRegBits);
+    const size_t bytes = size_t(rows) * (rows <= 128 ? threads / 16 : threads / 64) * sizeof(double);
+    if (bytes > kAdjAccBytesMax)
+      return fail(QSV_EUNSUP, "adjoint pass with " + std::to_string(rows) + " generator rows exceeds LDS");
+  }
+  return QSV_OK;
+}
+
+int qtile_jit_build_adjoint(const QProgram& qp, bool c128, QJit& out) {
+  QSV_TRY(qtile_jit_adjoint_check(qp));
+  return build_modules(qp, "qta_",
+                       [&](size_t p0, size_t p1) { return qtile_jit_adjoint_source(qp, c128, p0, p1); }, out);
+}
+
+int qtile_jit_launch_adjoint(const QJit& j, size_t i, void* phi, void* lam, const QOp* ops, double* part,
+                             hipStream_t st) {
+  if (i >= j.fn.size()) return fail(QSV_EINVAL, "adjoint pass index out of range");
+  uint64_t ntiles = j.ntiles[i];
+  const void* cf = ops;
+  void* args[] = {&phi, &lam, &cf, &ntiles, &part};
+  QSV_CHECK_HIP(hipModuleLaunchKernel(j.fn[i], kAdjPartStride, 1, 1, j.threads, 1, 1, 0, st, args, nullptr));
   return QSV_OK;
 }
 

@@ -57,6 +57,19 @@ int qtile_jit_compile(const std::string& src, std::vector<char>& code);
 // True when hiprtc loads, so compiled passes can be built.
 bool qtile_jit_available();
 
+// Adjoint reverse sweep: the passes of a program whose ops include Q_GEN,
+// each compiled as a kernel over two states (phi and lam).  A launch runs
+// kAdjPartStride blocks; block b writes the FP64 partial sum of pass-local
+// row r at part[2 * (r * kAdjPartStride + b)] (imaginary slot zero), so the
+// caller points `part` at its first row of a rows x kAdjPartStride table.
+constexpr int kAdjPartStride = 1024;
+constexpr size_t kAdjAccBytesMax = 64 * 1024;  // LDS row accumulators of one block
+std::string qtile_jit_adjoint_source(const QProgram& qp, bool c128, size_t p0, size_t p1);
+int qtile_jit_adjoint_check(const QProgram& qp);
+int qtile_jit_build_adjoint(const QProgram& qp, bool c128, QJit& out);
+int qtile_jit_launch_adjoint(const QJit& j, size_t i, void* phi, void* lam, const QOp* ops, double* part,
+                             hipStream_t st);
+
 // Module groups found in the process-wide cache, loaded from the on-disk
 // cache, and compiled, since the library was loaded.
 void qtile_jit_cache_stats(int64_t* mem_hits, int64_t* disk_hits, int64_t* compiles);

@@ -86,14 +86,26 @@ def test_adjoint_vs_param_shift_tfim(fuse_k):
     assert abs(back[0] - 1) < 1e-12 and np.abs(back[1:]).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [6, 15])
-def test_adjoint_all_differentiable_kinds_vs_fd(n):
-    """Every differentiable kind against central differences.  At n=15 the
-    reverse sweep's tiles (2^11 amplitudes) leave qubits outside the tile, so
-    generators and controls on address bits outside it are covered."""
+@pytest.fixture
+def adjoint_form(monkeypatch):
+    """Select the reverse sweep: compiled two-state passes ("jit") or the
+    interpreted LDS passes ("interp")."""
+    def use(form):
+        monkeypatch.setenv("QSV_ADJOINT", form)
+    return use
+
+
+@pytest.mark.parametrize("n,form", [(6, "interp"), (15, "interp"), (12, "jit"), (15, "jit")])
+def test_adjoint_all_differentiable_kinds_vs_fd(n, form, adjoint_form):
+    """Every differentiable kind against central differences, through both
+    reverse sweeps.  At n=15 the interpreted sweep's tiles (2^11 amplitudes)
+    and the compiled sweep's (2^12) leave qubits outside the tile, so
+    generators and controls on address bits outside it are covered; at
+    n=12 the compiled sweep's one tile is the whole state."""
     import torch
     from qclabpp_amd.autograd import finite_difference
-    print(f"adjoint kinds n={n}", flush=True)
+    adjoint_form(form)
+    print(f"adjoint kinds n={n} form={form}", flush=True)
     rs = np.random.default_rng(4)
     circ = [G.Hadamard(q) for q in range(n)]
     p = 0
@@ -122,12 +134,15 @@ def test_adjoint_all_differentiable_kinds_vs_fd(n):
     assert np.abs(g - fd).max() < 1e-8
 
 
-def test_adjoint_pass_with_many_parametrised_gates_vs_fd():
+@pytest.mark.parametrize("form", ["interp", "jit"])
+def test_adjoint_pass_with_many_parametrised_gates_vs_fd(form, adjoint_form):
     """384 parametrised gates on 11 qubits of a 13-qubit state fit one tile
-    pass of the reverse sweep, which then runs in several launches of at
-    most 128 gradient rows each."""
+    pass of the reverse sweep: the interpreted sweep runs it in several
+    launches of at most 128 gradient rows each, the compiled one in a single
+    launch whose rows reduce over whole waves (one LDS slot per wave)."""
     import torch
     from qclabpp_amd.autograd import finite_difference
+    adjoint_form(form)
     n = 13
     rs = np.random.default_rng(12)
     circ = [G.Hadamard(q) for q in range(n)]
@@ -197,3 +212,51 @@ def test_adjoint_complex64_vs_complex128(n):
     (e128, g128), (e64, g64) = out[torch.complex128], out[torch.complex64]
     assert abs(e64 - e128) < 1e-5
     assert np.abs(np.asarray(g64) - np.asarray(g128)).max() < 1e-4
+
+
+@pytest.mark.parametrize("dt,tol", [("c128", 1e-12), ("c64", 2e-4)])
+@pytest.mark.parametrize("n", [12, 14, 17])
+def test_adjoint_compiled_matches_interpreted(n, dt, tol, adjoint_form):
+    """The compiled reverse sweep (two states per pass, generator rows
+    reduced per DPP row into LDS slots) against the interpreted one on every
+    differentiable kind with fixed gates between them (permutations fold
+    into the register map, so generators see relabeled registers); psi is
+    rewound by both."""
+    import torch
+    from tests.test_jit_compile import differentiable_circuit
+    print(f"adjoint compiled vs interpreted n={n} {dt}", flush=True)
+    circ, p = differentiable_circuit(n, n)
+    # the TFIM ansatz after it, its parameters shared with the first ones
+    circ += W.tfim_ansatz(n, 2, seed=n)
+    H = mixed_H(n)
+    cdt = torch.complex128 if dt == "c128" else torch.complex64
+    res = {}
+    for form in ("interp", "jit"):
+        adjoint_form(form)
+        psi = E.new_state(n, cdt)
+        lam = torch.empty_like(psi)
+        e, g = O.adjoint_gradient(psi, lam, circ, H)
+        back = psi.cpu().numpy()
+        assert abs(back[0] - 1) < (1e-11 if dt == "c128" else 1e-4)
+        res[form] = (e, np.asarray(g))
+    (ei, gi), (ej, gj) = res["interp"], res["jit"]
+    scale = max(1.0, float(np.abs(gi).max()))
+    assert abs(ei - ej) < tol * max(1.0, abs(ei))
+    assert np.abs(gi - gj).max() < tol * scale, np.abs(gi - gj).max()
+
+
+@pytest.mark.parametrize("n", [12, 14])
+def test_adjoint_compiled_vs_param_shift(n, adjoint_form):
+    """The compiled reverse sweep against the oracle's CPU parameter shift
+    on the TFIM ansatz (every parameter)."""
+    import torch
+    adjoint_form("jit")
+    circ = W.tfim_ansatz(n, 2, seed=99)
+    theta = np.array(W.params_of(circ))
+    H = O.tfim(n)
+    psi = E.new_state(n)
+    lam = torch.empty_like(psi)
+    e, g = O.adjoint_gradient(psi, lam, circ, H)
+    assert abs(e - oracle.energy(n, circ, H, theta)) < 1e-12
+    ps = oracle.param_shift(n, circ, H, theta, list(range(theta.size)))
+    assert np.abs(g - ps).max() < 1e-12

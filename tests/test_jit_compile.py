@@ -126,3 +126,59 @@ def test_relabeling_schedules_compile(dtype):
     compiles."""
     n = 20
     assert _check(n, W.brickwork(n, 20, seed=50) + every_kind(n, 20), dtype) >= 2
+
+
+def _adjoint_compile(n, circ, dtype):
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("engine library not built")
+    arr, cnt = G.records(circ)
+    size = C.c_size_t()
+    npass = C.c_int64()
+    rows = C.c_int64()
+    st = _lib.lib().qsv_adjoint_compile_check(n, dtype, arr, cnt, C.byref(size), C.byref(npass),
+                                              C.byref(rows))
+    if st == _lib.QSV_EUNSUP and b"hiprtc" in _lib.lib().qsv_last_error():
+        pytest.skip("hiprtc not loadable here")
+    _lib.check(st)
+    return npass.value, rows.value, size.value
+
+
+def differentiable_circuit(n, seed):
+    """Every differentiable kind (both control states, generators on
+    scattered qubits), shared parameters, and fixed gates between them."""
+    import numpy as np
+    rs = np.random.default_rng(seed)
+    circ = [G.Hadamard(q) for q in range(n)]
+    p = 0
+    for ctor in (G.RotationX, G.RotationY, G.RotationZ, G.Phase):
+        for q in range(n):
+            circ.append(ctor(q, rs.uniform(-3, 3), param=p))
+            p += 1
+    circ += [G.CX(0, n - 1), G.SWAP(1, 4), G.iSWAP(2, n - 2), G.PauliY(3)]
+    for ctor in (G.RotationXX, G.RotationYY, G.RotationZZ):
+        for q in range(n - 1):
+            circ.append(ctor(q, q + 1, rs.uniform(-3, 3), param=p))
+            p += 1
+        circ.append(ctor(0, n - 1, rs.uniform(-3, 3), param=p))
+        p += 1
+    for ctor in (G.CRotationX, G.CRotationY, G.CRotationZ, G.CPhase):
+        for cs in (1, 0):
+            for c, t in ((2, 4), (5, 1), (0, n - 1), (n - 1, 3)):
+                circ.append(ctor(c, t, rs.uniform(-3, 3), controlState=cs, param=p))
+                p += 1
+    circ.append(G.RotationY(0, 0.7, param=0))  # a shared parameter
+    return circ, p
+
+
+@pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
+def test_adjoint_passes_compile(dtype):
+    """The compiled reverse sweep (two states, generator rows) generates and
+    compiles for every differentiable kind and for the TFIM ansatz; every
+    parametrised gate gets one row."""
+    circ, p = differentiable_circuit(14, 3)
+    npass, rows, size = _adjoint_compile(14, circ, dtype)
+    assert npass >= 1 and size > 0
+    assert rows == sum(1 for g in circ if g.param >= 0)
+    tf = W.tfim_ansatz(16, 3)
+    npass, rows, size = _adjoint_compile(16, tf, dtype)
+    assert rows == len(tf) and npass >= 2
