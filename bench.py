@@ -61,6 +61,9 @@ def parse():
                     help="time the std::vector entry (PCIe in and out) on the main workload (0 skips it)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the dominant kernels and a 16 GiB copy (for rocprofv3 --pmc)")
+    ap.add_argument("--legs", default="",
+                    help="comma list of side legs to run alone, without the main leg "
+                         "(adjoint, adjoint_big, batch, precision; for profiling runs)")
     return ap.parse_args()
 
 
@@ -129,20 +132,22 @@ def adjoint_leg(args):
     H = O.tfim(n)
     psi = E.new_state(n)
     lam = torch.empty_like(psi)
-    O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)  # warm-up (compiles the forward passes)
+    O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)  # warm-up (compiles the passes)
     E.set_basis(psi, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e_adj, g_adj = O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
+    prof = adjoint_roofline(O.adjoint_profile(), n)
     del psi, lam
     t0 = time.perf_counter()
     g_fd = finite_difference(circ, H, theta, n, h=1e-6, fuse_k=E.AUTO)
     torch.cuda.synchronize()
     t_fd = time.perf_counter() - t0
     out = {"workload": f"tfim ansatz n={n} layers={L} params={theta.size} complex128",
-           "energy": e_adj, "adjoint_s": t_adj, "fd_s": t_fd, "fd_over_adjoint": t_fd / t_adj,
+           "energy": e_adj, "adjoint_s": t_adj, "phases": prof, "fd_s": t_fd,
+           "fd_over_adjoint": t_fd / t_adj,
            "max_abs_adjoint_minus_fd": float(np.abs(g_adj - g_fd).max()),
            "max_abs_grad": float(np.abs(g_adj).max())}
     if args.ps_params > 0 and not args.no_cpu:
@@ -184,11 +189,17 @@ def adjoint_big_leg(args):
     H = O.tfim(n)
     psi = E.new_state(n)
     lam = torch.empty_like(psi)
+    t0 = time.perf_counter()
+    O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)  # warm-up (compiles the passes)
+    torch.cuda.synchronize()
+    first_s = time.perf_counter() - t0
+    E.set_basis(psi, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e_adj, g_adj = O.adjoint_gradient(psi, lam, circ, H, fuse_k=E.AUTO)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
+    prof = adjoint_roofline(O.adjoint_profile(), n)
     del psi, lam
     torch.cuda.empty_cache()
     which = [0, theta.size - 1]
@@ -198,8 +209,26 @@ def adjoint_big_leg(args):
     torch.cuda.empty_cache()
     return {"workload": f"tfim ansatz n={n} layers={L} params={theta.size} complex128",
             "state_bytes": (1 << n) * 16, "energy": e_adj, "adjoint_s": t_adj,
+            "first_call_s": first_s, "phases": prof,
             "fd_params": which, "fd_s_for_those": t_fd,
             "max_abs_adjoint_minus_fd": float(np.abs(g_adj[which] - g_fd).max())}
+
+
+def adjoint_roofline(prof, n, elem=16):
+    """The adjoint's phases with the reverse sweep's roofline: each reverse
+    launch reads and writes both states once, 4 * 2^n * 16 B algorithmic,
+    over the mean launch time (HIP events around the launches)."""
+    out = dict(prof)
+    launches = prof["reverse_launches"]
+    if launches:
+        per = prof["reverse_ms"] / launches
+        alg = 4 * (1 << n) * elem
+        out["reverse_roofline"] = {"bound": "hbm", "achieved": alg / (per / 1e3) / 1e9, "peak": PEAK_HBM,
+                                   "unit": "GB/s", "frac": alg / (per / 1e3) / 1e9 / PEAK_HBM,
+                                   "alg_bytes_per_launch": alg, "ms_per_launch": per,
+                                   "kernel": "qta_* (compiled two-state reverse passes)" if prof["compiled"]
+                                   else "k_adjoint_tiles (interpreted)"}
+    return out
 
 
 def host_entry_leg(args, circ, ms_jit):
@@ -448,6 +477,20 @@ def main():
     if args.kernels_only:
         return kernels_only(args.n)
     import torch
+
+    if args.legs:
+        legs = {"adjoint": adjoint_leg, "adjoint_big": adjoint_big_leg, "batch": batch_leg,
+                "precision": precision_leg}
+        out = {}
+        for name in args.legs.split(","):
+            if name not in legs:
+                print(f"unknown leg {name!r}; choose from {sorted(legs)}", file=sys.stderr)
+                return 2
+            progress(name)
+            out[name] = legs[name](args)
+            torch.cuda.empty_cache()
+        print(json.dumps(out))
+        return 0
 
     from qclabpp_amd import engine as E
     from qclabpp_amd import gates as G

@@ -253,13 +253,31 @@ int qsv_dev_apply_pauli_sum(const void* in, void* out, int n, int dtype,
  *   lam    : scratch state of the same size
  *   theta  : nparams angles overriding each parametrised gate's theta
  *            (gate.param indexes theta), or NULL to use the records' own
- *   fuse_k : fusion width of the forward pass (0 = plain)
+ *   fuse_k : form of the forward pass (a qsv_program_create fuse_k;
+ *            QSV_AUTO picks the fastest)
  *   energy : E;  grad[p] = dE/dtheta_p summed over the gates sharing p
- * Differentiable kinds: RX RY RZ RXX RYY RZZ CRX CRY CRZ P CP, Op 'N'.    */
+ * Differentiable kinds: RX RY RZ RXX RYY RZZ CRX CRY CRZ P CP, Op 'N'.
+ * The reverse sweep runs as passes compiled for the circuit's structure
+ * over both states from 12 qubits up where hiprtc loads (the schedule and
+ * kernels cached like QSV_TILED_JIT programs), else as interpreted LDS
+ * tile passes; QSV_ADJOINT=interp|jit in the environment forces one.      */
 int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_gate* gates,
                              size_t ngates, const double* theta, size_t nparams,
                              const qsv_pauli_term* terms, size_t nterms, int fuse_k,
                              double* energy, double* grad, void* stream);
+/* GPU time of the last qsv_dev_adjoint_gradient call, by phase (HIP events
+ * on its stream): the forward pass; lam = H psi and the energy; the reverse
+ * sweep's launches alone (from the first to the last reverse kernel, no host
+ * work in between).  `compiled` is 1 when the reverse sweep ran as compiled
+ * two-state passes, 0 for the interpreted LDS passes.  Each reverse launch
+ * reads and writes both states once (4 * 2^n * sizeof(complex)).  No
+ * counterpart in the reference.                                            */
+typedef struct qsv_adjoint_profile {
+  double forward_ms, hamiltonian_ms, reverse_ms;
+  int64_t reverse_launches;
+  int32_t compiled, reserved;
+} qsv_adjoint_profile;
+int qsv_adjoint_profile_get(qsv_adjoint_profile* out);
 /* Build check of the compiled reverse sweep without a GPU: the inverse
  * gates, last first, scheduled into two-state tiled passes with each
  * parametrised gate's generator measured before it; every pass generated

@@ -85,6 +85,13 @@ def cache_stats() -> dict:
     return {k: getattr(st, k) for k, _ in QsvCacheStats._fields_}
 
 
+class QsvAdjointProfile(C.Structure):
+    """Mirror of `qsv_adjoint_profile` (include/qsv.h)."""
+
+    _fields_ = [("forward_ms", C.c_double), ("hamiltonian_ms", C.c_double), ("reverse_ms", C.c_double),
+                ("reverse_launches", C.c_int64), ("compiled", C.c_int32), ("reserved", C.c_int32)]
+
+
 class QsvProgramInfo(C.Structure):
     """Mirror of `qsv_program_info` (include/qsv.h)."""
 
@@ -184,6 +191,7 @@ SIGNATURES = {
     "qsv_host_circuit_matrix": (_I, [_P, _I, _I, _GP, _SZ, _I]),
     "qsv_dev_nrmf": (_I, [_P, _P, _U64, _I, _DP, _P]),
     "qsv_circuit_nrmf": (_I, [_I, _I, _GP, _SZ, _GP, _SZ, _I, _DP]),
+    "qsv_adjoint_profile_get": (_I, [C.POINTER(QsvAdjointProfile)]),
     "qsv_adjoint_compile_check": (_I, [_I, _I, _GP, _SZ, C.POINTER(_SZ), C.POINTER(C.c_int64),
                                        C.POINTER(C.c_int64)]),
     "qsv_dev_adjoint_gradient": (_I, [_P, _P, _I, _I, _GP, _SZ, _DP, _SZ, C.POINTER(QsvPauliTerm),

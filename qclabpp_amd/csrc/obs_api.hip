@@ -150,6 +150,28 @@ bool generator(const qsv_gate& g, int n, PauliOp& G, double& wr, double& wi) {
 
 static_assert(kAdjPartStride == kRedBlocks, "compiled adjoint passes write reduction-row partials");
 
+// Phase timing of the last adjoint call (HIP events on its stream).
+struct AdjProfile {
+  std::mutex mu;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int dev = -1;
+  bool valid = false;
+  int64_t reverse_launches = 0;
+  int compiled = 0;
+  int record(int i, hipStream_t st) {
+    int d = 0;
+    QSV_CHECK_HIP(hipGetDevice(&d));
+    if (d != dev) {
+      for (hipEvent_t& e : ev)
+        if (e) (void)hipEventDestroy(e), e = nullptr;
+      dev = d;
+    }
+    if (!ev[i]) QSV_CHECK_HIP(hipEventCreate(&ev[i]));
+    QSV_CHECK_HIP(hipEventRecord(ev[i], st));
+    return QSV_OK;
+  }
+} g_adjprof;
+
 // Tile bits of the compiled reverse sweep: one shared LDS tile of 2^12
 // amplitudes (64 KiB for complex128) that both states pass through.
 constexpr int kAdjJitTileBits = 12;
@@ -222,7 +244,8 @@ int adjoint_schedule(int n, int dtype, const std::vector<qsv_gate>& gs, QProgram
 
 int adjoint_reverse_jit(void* psi, void* lam, int n, int dtype, const std::vector<qsv_gate>& gs,
                         double2* partials, int row0, std::vector<int>& row_param,
-                        std::vector<double>& row_scale, hipStream_t st, bool& done) {
+                        std::vector<double>& row_scale, hipStream_t st, bool& done,
+                        int64_t& launches) {
   done = false;
   QProgram qp;
   std::vector<int> pass_row0;
@@ -237,10 +260,13 @@ int adjoint_reverse_jit(void* psi, void* lam, int n, int dtype, const std::vecto
   QSV_TRY(qtile_jit_build_adjoint(qp, dtype == QSV_C128, qj));
   DevArray<QOp> d_ops;
   QSV_TRY(d_ops.upload(qp.ops));
+  QSV_TRY(g_adjprof.record(2, st));
   for (size_t i = 0; i < qp.passes.size(); ++i) {
     double* part = reinterpret_cast<double*>(partials + size_t(row0 + pass_row0[i]) * kRedBlocks);
     QSV_TRY(qtile_jit_launch_adjoint(qj, i, psi, lam, d_ops.p, part, st));
   }
+  QSV_TRY(g_adjprof.record(3, st));
+  launches = int64_t(qp.passes.size());
   // the op table is freed with d_ops: finish before returning
   QSV_CHECK_HIP(hipStreamSynchronize(st));
   done = true;
@@ -362,6 +388,20 @@ int qsv_dev_apply_pauli_sum(const void* in, void* out, int n, int dtype,
   return QSV_OK;
 }
 
+int qsv_adjoint_profile_get(qsv_adjoint_profile* out) {
+  if (!out) return fail(QSV_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(g_adjprof.mu);
+  if (!g_adjprof.valid) return fail(QSV_EINVAL, "no adjoint call to report");
+  float ms[3];
+  for (int i = 0; i < 3; ++i) QSV_CHECK_HIP(hipEventElapsedTime(&ms[i], g_adjprof.ev[i], g_adjprof.ev[i + 1]));
+  out->forward_ms = ms[0];
+  out->hamiltonian_ms = ms[1];
+  out->reverse_ms = ms[2];
+  out->reverse_launches = g_adjprof.reverse_launches;
+  out->compiled = g_adjprof.compiled;
+  return QSV_OK;
+}
+
 int qsv_adjoint_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
                               size_t* code_bytes, int64_t* npasses, int64_t* nrows) {
   if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
@@ -417,7 +457,11 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
   // forward: phi = U psi0
   Program fwd;
   QSV_TRY(fwd.build(n, dtype, gs.data(), gs.size(), fuse_k, 0));
+  std::lock_guard<std::mutex> plk(g_adjprof.mu);
+  g_adjprof.valid = false;
+  QSV_TRY(g_adjprof.record(0, st));
   QSV_TRY(fwd.run(psi, st));
+  QSV_TRY(g_adjprof.record(1, st));
   // lam = H phi
   QSV_TRY(qsv_dev_apply_pauli_sum(psi, lam, n, dtype, terms, nterms, stream));
   // per-gate generators and un-apply ops
@@ -440,8 +484,11 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
     std::vector<int> rp;
     std::vector<double> rs;
     bool done = false;
-    QSV_TRY(adjoint_reverse_jit(psi, lam, n, dtype, gs, partials, 1, rp, rs, st, done));
+    QSV_TRY(adjoint_reverse_jit(psi, lam, n, dtype, gs, partials, 1, rp, rs, st, done,
+                                g_adjprof.reverse_launches));
     if (done) {
+      g_adjprof.compiled = 1;
+      g_adjprof.valid = true;
       const int rows = 1 + int(rp.size());
       QSV_TRY(launch_reduce_rows(partials, rows, red, st));
       std::vector<double2> host(rows);
@@ -572,6 +619,7 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
   DevArray<TGen> d_gens;
   QSV_TRY(d_tops.upload(tops));
   QSV_TRY(d_gens.upload(gens));
+  QSV_TRY(g_adjprof.record(2, st));
   for (size_t r = 0; r < runs.size(); ++r) {
     void* part = partials + size_t(run_row0[r]) * kRedBlocks;
     const TOp* o = d_tops.p + runs[r].op_begin;
@@ -580,6 +628,10 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
                 ? launch_adjoint_tiles<double>(psi, lam, runs[r], o, g, run_rows[r], part, kRedBlocks, st)
                 : launch_adjoint_tiles<float>(psi, lam, runs[r], o, g, run_rows[r], part, kRedBlocks, st));
   }
+  QSV_TRY(g_adjprof.record(3, st));
+  g_adjprof.compiled = 0;
+  g_adjprof.reverse_launches = int64_t(runs.size());
+  g_adjprof.valid = true;
   QSV_TRY(launch_reduce_rows(partials, rows, red, st));
   std::vector<double2> host(rows);
   QSV_CHECK_HIP(hipMemcpyAsync(host.data(), red, rows * sizeof(double2), hipMemcpyDeviceToHost, st));

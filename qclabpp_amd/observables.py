@@ -125,3 +125,15 @@ def adjoint_gradient(psi, lam, gates, H: PauliSum, theta=None, nparams=None, fus
                                          arr, cnt, th, nparams, harr, hcnt, fuse_k, C.byref(e), g,
                                          stream_handle(stream)))
     return e.value, np.array(g[:nparams])
+
+
+def adjoint_profile() -> dict:
+    """GPU milliseconds of the last adjoint_gradient call by phase
+    (qsv_adjoint_profile_get): forward, hamiltonian (lam = H psi and E),
+    reverse (the reverse sweep's launches), with the launch count and
+    whether the compiled two-state passes ran."""
+    from ._lib import QsvAdjointProfile
+    p = QsvAdjointProfile()
+    check(lib().qsv_adjoint_profile_get(C.byref(p)))
+    return {"forward_ms": p.forward_ms, "hamiltonian_ms": p.hamiltonian_ms, "reverse_ms": p.reverse_ms,
+            "reverse_launches": p.reverse_launches, "compiled": bool(p.compiled)}

@@ -236,6 +236,9 @@ def test_adjoint_compiled_matches_interpreted(n, dt, tol, adjoint_form):
         psi = E.new_state(n, cdt)
         lam = torch.empty_like(psi)
         e, g = O.adjoint_gradient(psi, lam, circ, H)
+        prof = O.adjoint_profile()
+        assert prof["compiled"] == (form == "jit") and prof["reverse_launches"] >= 1
+        assert prof["forward_ms"] > 0 and prof["reverse_ms"] > 0
         back = psi.cpu().numpy()
         assert abs(back[0] - 1) < (1e-11 if dt == "c128" else 1e-4)
         res[form] = (e, np.asarray(g))
