@@ -11,6 +11,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstddef>
 #include <cstdio>
@@ -509,25 +510,92 @@ int adj_pass_rows(const QProgram& qp, size_t pi) {
   return rows;
 }
 
-void emit_gen(std::ostringstream& s, const QOp& o, bool c128, int slots_per_wave) {
+// The weights k_p of one generator share their modulus (the generator's
+// coefficients are 0 or of modulus 1, times one weight), which is taken
+// out of the sum; each term then costs two FMAs with signs folded in:
+//   Re(k conj(l) f) = kr (l.x f.x + l.y f.y) - ki (l.x f.y - l.y f.x).
+double gen_modulus(const QOp& o) {
+  for (int p = 0; p < NR; ++p)
+    if (o.act >> p & 1u) return std::hypot(double(o.mf[2 * p]), double(o.mf[2 * p + 1]));
+  return 1.0;
+}
+
+std::string lit(double x) {
+  std::ostringstream e;
+  e.precision(17);
+  e << std::showpoint << x;
+  return e.str();
+}
+
+void emit_gen_sum(std::ostringstream& s, const QOp& o, int spw) {
+  const double m = gen_modulus(o);
+  if (m != 1.0) s << "      t *= " << lit(m) << ";\n";
+  s << "      qta_acc" << spw << "(t, acc + " << o.a << " * NSLOT, tid);\n";
+}
+
+void emit_gen(std::ostringstream& s, const QOp& o, bool c128, int spw) {
   const char* cv = c128 ? "" : "(double)";
+  const double m = gen_modulus(o);
   s << "    {  // generator row " << o.a << "\n"
     << "      double t = 0.0;\n";
   for (int p = 0; p < NR; ++p) {
     if (!(o.act >> p & 1u)) continue;
     const int q = p ^ o.d;
-    const double kr = o.mf[2 * p], ki = o.mf[2 * p + 1];
-    const std::string l = "w[" + std::to_string(q) + "]", f = "v[" + std::to_string(p) + "]";
-    // Re(k conj(l) f) = kr (l.x f.x + l.y f.y) - ki (l.x f.y - l.y f.x)
+    const double kr = o.mf[2 * p] / m, ki = o.mf[2 * p + 1] / m;  // 0 or +-1
+    const std::string lx = std::string(cv) + "w[" + std::to_string(q) + "].x",
+                      ly = std::string(cv) + "w[" + std::to_string(q) + "].y",
+                      fx = std::string(cv) + "v[" + std::to_string(p) + "].x",
+                      fy = std::string(cv) + "v[" + std::to_string(p) + "].y";
+    auto sg = [](double k) { return k > 0 ? "" : "-"; };
     if (kr != 0)
-      s << "      t = fma(" << std::to_string(kr) << ", " << cv << l << ".x * " << cv << f << ".x + " << cv
-        << l << ".y * " << cv << f << ".y, t);\n";
+      s << "      t = fma(" << sg(kr) << lx << ", " << fx << ", t);\n"
+        << "      t = fma(" << sg(kr) << ly << ", " << fy << ", t);\n";
     if (ki != 0)
-      s << "      t = fma(" << std::to_string(-ki) << ", " << cv << l << ".x * " << cv << f << ".y - " << cv
-        << l << ".y * " << cv << f << ".x, t);\n";
+      s << "      t = fma(" << sg(-ki) << lx << ", " << fy << ", t);\n"
+        << "      t = fma(" << sg(ki) << ly << ", " << fx << ", t);\n";
   }
-  s << "      qta_acc" << slots_per_wave << "(t, acc + " << o.a << " * NSLOT, tid);\n"
-    << "    }\n";
+  emit_gen_sum(s, o, spw);
+  s << "    }\n";
+}
+
+// A run of generators with d = 0 (Z strings and projectors: RZ, RZZ, CRZ,
+// P, CP) with no op between them sees the same registers, and each one's
+// weights are imaginary: k_p = i kappa_p, Re(k conj(l) f) = -kappa_p u_p with
+// u_p = Im(conj(l) f) = l.x f.y - l.y f.x.  The u_p are formed once for the
+// run and each generator adds them with its signs.
+bool diag_gen(const QOp& o) {
+  if (o.kind != Q_GEN || o.d != 0) return false;
+  for (int p = 0; p < NR; ++p)
+    if ((o.act >> p & 1u) && o.mf[2 * p] != 0) return false;
+  return true;
+}
+
+void emit_diag_gen_run(std::ostringstream& s, const QProgram& qp, int o0, int o1, bool c128, int spw) {
+  const char* cv = c128 ? "" : "(double)";
+  uint32_t used = 0;
+  for (int o = o0; o < o1; ++o) used |= qp.ops[o].act;
+  s << "    {  // generators " << qp.ops[o0].a << ".." << qp.ops[o1 - 1].a << ": one diagonal product\n"
+    << "      double u[16];\n";
+  for (int p = 0; p < NR; ++p)
+    if (used >> p & 1u) {
+      const std::string i = std::to_string(p);
+      s << "      u[" << i << "] = fma(" << cv << "w[" << i << "].x, " << cv << "v[" << i << "].y, -(" << cv << "w["
+        << i << "].y * " << cv << "v[" << i << "].x));\n";
+    }
+  for (int o = o0; o < o1; ++o) {
+    const QOp& g = qp.ops[o];
+    const double m = gen_modulus(g);
+    s << "      {  // generator row " << g.a << "\n"
+      << "        double t = 0.0";
+    for (int p = 0; p < NR; ++p)
+      if (g.act >> p & 1u) s << (g.mf[2 * p + 1] / m > 0 ? " - " : " + ") << "u[" << p << "]";
+    s << ";\n";
+    std::ostringstream tail;
+    emit_gen_sum(tail, g, spw);
+    s << "  " << tail.str();
+    s << "      }\n";
+  }
+  s << "    }\n";
 }
 
 void emit_adj_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) {
@@ -656,6 +724,15 @@ void emit_adj_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c1
       << "      const QC R* cf = cf0;\n"
       << "      asm volatile(\"\" : \"+s\"(cf));\n";
     for (int o = Q.op_begin; o < Q.op_end; ++o) {
+      if (diag_gen(qp.ops[o])) {
+        int e = o + 1;
+        while (e < Q.op_end && diag_gen(qp.ops[e])) ++e;
+        if (e - o >= 2) {
+          emit_diag_gen_run(s, qp, o, e, c128, spw);
+          o = e - 1;
+          continue;
+        }
+      }
       if (qp.ops[o].kind == Q_GEN) {
         emit_gen(s, qp.ops[o], c128, spw);
       } else {
