@@ -126,7 +126,10 @@ __device__ __forceinline__ void u_d2(C& a0, C& a1, C& a2, C& a3, const QC R* cf,
 }
 )QSV";
 
-void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128, const char* arr = "v") {
+// Op k applied to the register arrays `arrs`, one after the other (the
+// data-dependent scaled forms inside one branch).
+void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128,
+             const std::vector<std::string>& arrs = {"v"}) {
   // index of coefficient slot 0 of op k in the QOp array read as R[]
   const size_t stride = c128 ? sizeof(QOp) / sizeof(double) : sizeof(QOp) / sizeof(float);
   const size_t base = k * stride + (c128 ? offsetof(QOp, m) / sizeof(double) : offsetof(QOp, mf) / sizeof(float));
@@ -136,31 +139,47 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128, const cha
     for (int p = 0; p < NR; ++p)
       if (!(p & K)) body(p, p ^ o.d, bool(o.roles >> p & 1), bool(o.act >> p & 1));
   };
-  auto v = [arr](int p) { return std::string(arr) + "[" + std::to_string(p) + "]"; };
+  std::string arr = arrs[0];
+  auto v = [&arr](int p) { return arr + "[" + std::to_string(p) + "]"; };
+  // body() once per array, with v() naming that array's registers
+  auto each = [&](auto body) {
+    for (const std::string& a : arrs) {
+      arr = a;
+      body();
+    }
+  };
   s << "    {\n";
   switch (o.kind) {
     case Q_DIAGR:
-      for (int p = 0; p < NR; ++p)
-        if (o.b >> p & 1) s << "      " << v(p) << " = cmul(cf2(cf, " << at(2 * p) << "), " << v(p) << ");\n";
+      each([&] {
+        for (int p = 0; p < NR; ++p)
+          if (o.b >> p & 1) s << "      " << v(p) << " = cmul(cf2(cf, " << at(2 * p) << "), " << v(p) << ");\n";
+      });
       break;
     case Q_ROTP:
       s << "      const R c = cf[" << at(0) << "], sn = cf[" << at(4) << "];\n";
-      pairs([&](int p, int q, bool sw, bool) {
-        s << "      u_rot(" << v(p) << ", " << v(q) << ", c, " << (sw ? "-sn" : "sn") << ");\n";
+      each([&] {
+        pairs([&](int p, int q, bool sw, bool) {
+          s << "      u_rot(" << v(p) << ", " << v(q) << ", c, " << (sw ? "-sn" : "sn") << ");\n";
+        });
       });
       break;
     case Q_REALP:
       s << "      const R m0 = cf[" << at(0) << "], m1 = cf[" << at(2) << "], m2 = cf[" << at(4)
         << "], m3 = cf[" << at(6) << "];\n";
-      pairs([&](int p, int q, bool sw, bool) {
-        s << "      u_real(" << v(p) << ", " << v(q) << (sw ? ", m3, m2, m1, m0);\n" : ", m0, m1, m2, m3);\n");
+      each([&] {
+        pairs([&](int p, int q, bool sw, bool) {
+          s << "      u_real(" << v(p) << ", " << v(q) << (sw ? ", m3, m2, m1, m0);\n" : ", m0, m1, m2, m3);\n");
+        });
       });
       break;
     case Q_XLIKEP:
       s << "      const R xp = cf[" << at(0) << "], xq = cf[" << at(3) << "], xr = cf[" << at(5)
         << "], xs = cf[" << at(6) << "];\n";
-      pairs([&](int p, int q, bool sw, bool) {
-        s << "      u_xl(" << v(p) << ", " << v(q) << (sw ? ", xs, xr, xq, xp);\n" : ", xp, xq, xr, xs);\n");
+      each([&] {
+        pairs([&](int p, int q, bool sw, bool) {
+          s << "      u_xl(" << v(p) << ", " << v(q) << (sw ? ", xs, xr, xq, xp);\n" : ", xp, xq, xr, xs);\n");
+        });
       });
       break;
     case Q_ROTS:
@@ -170,14 +189,18 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128, const cha
       const bool rot = o.kind == Q_ROTS;
       s << "      const R k = cf[" << at(0) << "];\n"
         << "      if (cf[" << at(2) << "] != R(0)) {\n";
-      pairs([&](int p, int q, bool sw, bool) {
-        if (rot) s << "        u_ta(" << v(p) << ", " << v(q) << (sw ? ", -k);\n" : ", k);\n");
-        else s << "        u_xa(" << v(p) << ", " << v(q) << ", k);\n";
+      each([&] {
+        pairs([&](int p, int q, bool sw, bool) {
+          if (rot) s << "        u_ta(" << v(p) << ", " << v(q) << (sw ? ", -k);\n" : ", k);\n");
+          else s << "        u_xa(" << v(p) << ", " << v(q) << ", k);\n";
+        });
       });
       s << "      } else {\n";
-      pairs([&](int p, int q, bool sw, bool) {
-        if (rot) s << "        " << (sw ? "u_tbs(" : "u_tb(") << v(p) << ", " << v(q) << ", k);\n";
-        else s << "        u_xb(" << v(p) << ", " << v(q) << ", k);\n";
+      each([&] {
+        pairs([&](int p, int q, bool sw, bool) {
+          if (rot) s << "        " << (sw ? "u_tbs(" : "u_tb(") << v(p) << ", " << v(q) << ", k);\n";
+          else s << "        u_xb(" << v(p) << ", " << v(q) << ", k);\n";
+        });
       });
       s << "      }\n";
       break;
@@ -186,17 +209,21 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128, const cha
     case Q_CPAIR:
       s << "      const C m0 = cf2(cf, " << at(0) << "), m1 = cf2(cf, " << at(2) << "), m2 = cf2(cf, "
         << at(4) << "), m3 = cf2(cf, " << at(6) << ");\n";
-      pairs([&](int p, int q, bool sw, bool on) {
-        if (o.kind == Q_CPAIR && !on) return;
-        s << "      u_pair(" << v(p) << ", " << v(q) << (sw ? ", m3, m2, m1, m0);\n" : ", m0, m1, m2, m3);\n");
+      each([&] {
+        pairs([&](int p, int q, bool sw, bool on) {
+          if (o.kind == Q_CPAIR && !on) return;
+          s << "      u_pair(" << v(p) << ", " << v(q) << (sw ? ", m3, m2, m1, m0);\n" : ", m0, m1, m2, m3);\n");
+        });
       });
       break;
     case Q_DENSE2: {
       const int A = 1 << o.a, B = 1 << o.b;
-      for (int i = 0; i < NR; ++i)
-        if (!(i & A) && !(i & B))
-          s << "      u_d2(" << v(i) << ", " << v(i | B) << ", " << v(i | A) << ", " << v(i | A | B)
-            << ", cf, " << at(0) << ");\n";
+      each([&] {
+        for (int i = 0; i < NR; ++i)
+          if (!(i & A) && !(i & B))
+            s << "      u_d2(" << v(i) << ", " << v(i | B) << ", " << v(i | A) << ", " << v(i | A | B)
+              << ", cf, " << at(0) << ");\n";
+      });
       break;
     }
     default:
@@ -469,8 +496,10 @@ __device__ __forceinline__ double qta_dbl(long long b) { return __builtin_bit_ca
 template <int CTRL>
 __device__ __forceinline__ double qta_dpp(double x) {
   const long long b = qta_bits(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xf, 0xf, false);
+  // every lane reads a valid lane of its own row (quad_perm, row_ror), so
+  // the old value is never used and needs no initialising move
+  const int lo = __builtin_amdgcn_mov_dpp(int(b), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), CTRL, 0xf, 0xf, true);
   return qta_dbl((long long)(unsigned)lo | ((long long)hi << 32));
 }
 // sum over the 16 lanes of each DPP row: quad_perm [1,0,3,2], [2,3,0,1],
@@ -736,8 +765,10 @@ void emit_adj_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c1
       if (qp.ops[o].kind == Q_GEN) {
         emit_gen(s, qp.ops[o], c128, spw);
       } else {
-        emit_op(s, qp.ops[o], size_t(o), c128, "v");
-        emit_op(s, qp.ops[o], size_t(o), c128, "w");
+        // one state after the other: a shared branch for the scaled forms
+        // holds both states' temporaries at once and spills
+        emit_op(s, qp.ops[o], size_t(o), c128, {"v"});
+        emit_op(s, qp.ops[o], size_t(o), c128, {"w"});
       }
     }
     s << "    }\n";
