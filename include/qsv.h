@@ -242,6 +242,16 @@ int qsv_program_destroy(qsv_program* prog);
  * *total = sum_t coeff_t values[t] (may be NULL).                          */
 int qsv_dev_expectation(const void* psi, int n, int dtype, const qsv_pauli_term* terms,
                         size_t nterms, double* values, double* total, void* stream);
+/* qsv_dev_expectation without any device allocation: the reduction
+ * partials live in a fixed array in the library's own device memory (one
+ * per GPU, 4 MiB, held until the process ends) instead of the library's
+ * growable scratch.  No counterpart in the reference; for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  At most 256 terms (QSV_EINVAL beyond).  Calls
+ * on one GPU share that array: calls that overlap on one GPU (other host
+ * threads, other streams) race.                                            */
+int qsv_dev_expectation_noalloc(const void* psi, int n, int dtype, const qsv_pauli_term* terms,
+                                size_t nterms, double* values, double* total, void* stream);
 /* <a|b> = sum conj(a_x) b_x. */
 int qsv_dev_inner(const void* a, const void* b, int n, int dtype, double* re, double* im,
                   void* stream);

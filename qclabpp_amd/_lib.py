@@ -175,6 +175,7 @@ SIGNATURES = {
     "qsv_host_simulate": (_I, [_P, _I, _I, _GP, _SZ, _I]),
     "qsv_dev_simulate": (_I, [_P, _I, _I, _GP, _SZ, _I, _P]),
     "qsv_dev_expectation": (_I, [_P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _DP, _DP, _P]),
+    "qsv_dev_expectation_noalloc": (_I, [_P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _DP, _DP, _P]),
     "qsv_dev_inner": (_I, [_P, _P, _I, _I, _DP, _DP, _P]),
     "qsv_dev_apply_pauli_sum": (_I, [_P, _P, _I, _I, C.POINTER(QsvPauliTerm), _SZ, _P]),
     "qsv_batch_create": (_I, [_I, _I, C.POINTER(_I), _GP, C.POINTER(_SZ), C.POINTER(QsvBatchOpts),

@@ -273,18 +273,23 @@ int adjoint_reverse_jit(void* psi, void* lam, int n, int dtype, const std::vecto
   return QSV_OK;
 }
 
+// Device scratch an expectation needs for `rows` terms: per-term block
+// partials, the reduced rows, and the diagonal-term table of one pass.
+inline size_t expectation_scratch_bytes(size_t rows) {
+  return rows * kRedBlocks * sizeof(double2) + rows * sizeof(double2) + 64 * sizeof(uint64_t);
+}
+
+// The expectation values with the caller's scratch (expectation_scratch_bytes
+// at `scratch`, every byte of which is written before it is read).
 template <typename R>
-int expectation_impl(const void* psi, int n, const std::vector<PauliOp>& ops, double* values,
-                     hipStream_t st) {
+int expectation_core(const void* psi, int n, const std::vector<PauliOp>& ops, double* values,
+                     void* scratch, hipStream_t st) {
   // diagonal, projector-free terms: 32 per pass; others: one pass each
   std::vector<int> diag, off;
   for (int i = 0; i < int(ops.size()); ++i)
     (ops[i].flip == 0 && ops[i].pmask == 0 ? diag : off).push_back(i);
   const size_t rows = ops.size();
-  const size_t part_bytes = rows * kRedBlocks * sizeof(double2);
-  std::lock_guard<std::mutex> lk(g_scratch.mu);
-  QSV_TRY(g_scratch.ensure(part_bytes + rows * sizeof(double2) + 64 * sizeof(uint64_t)));
-  auto* partials = static_cast<double2*>(g_scratch.p);
+  auto* partials = static_cast<double2*>(scratch);
   auto* red = partials + rows * kRedBlocks;
   void* tmp = red + rows;
   // rows are laid out in the order: diagonal chunks, then off-diagonal terms
@@ -316,6 +321,49 @@ int expectation_impl(const void* psi, int n, const std::vector<PauliOp>& ops, do
   return QSV_OK;
 }
 
+template <typename R>
+int expectation_impl(const void* psi, int n, const std::vector<PauliOp>& ops, double* values,
+                     hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_scratch.mu);
+  QSV_TRY(g_scratch.ensure(expectation_scratch_bytes(ops.size())));
+  return expectation_core<R>(psi, n, ops, values, g_scratch.p, st);
+}
+
+// ---- the allocation-free expectation ------------------------------------
+// A fixed scratch in the library's own device memory (one copy per GPU,
+// held until the process ends), for callers that cannot allocate.  Host
+// code never uses the array's name as a device address: it takes the
+// address hipGetSymbolAddress reports, once per device, with that device
+// current.
+constexpr int kNoAllocMaxTerms = 256;
+constexpr size_t kNoAllocScratchBytes = (size_t(kNoAllocMaxTerms) * kRedBlocks + kNoAllocMaxTerms) * 16 + 64 * 8;
+static_assert(kNoAllocScratchBytes == (size_t(kNoAllocMaxTerms) * kRedBlocks + kNoAllocMaxTerms) * sizeof(double2) +
+                                          64 * sizeof(uint64_t),
+              "scratch layout of expectation_core");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocScratch {
+  std::mutex mu;
+  void* ptr[64] = {};  // device address of `scratch`, per device
+  int get(void** out) {
+    int dev = 0;
+    QSV_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(QSV_ENODEV, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return fail(QSV_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      }
+      ptr[dev] = p;
+    }
+    *out = ptr[dev];
+    return QSV_OK;
+  }
+} g_noalloc;
+
 }  // namespace
 }  // namespace qsv
 
@@ -333,6 +381,30 @@ int qsv_dev_expectation(const void* psi, int n, int dtype, const qsv_pauli_term*
   if (nterms)
     QSV_TRY(dtype == QSV_C128 ? expectation_impl<double>(psi, n, ops, v.data(), st)
                               : expectation_impl<float>(psi, n, ops, v.data(), st));
+  double s = 0;
+  for (size_t i = 0; i < nterms; ++i) {
+    if (values) values[i] = v[i];
+    s += ops[i].coeff * v[i];
+  }
+  if (total) *total = s;
+  return QSV_OK;
+}
+
+int qsv_dev_expectation_noalloc(const void* psi, int n, int dtype, const qsv_pauli_term* terms,
+                                size_t nterms, double* values, double* total, void* stream) {
+  QSV_TRY(check_state(psi, n, dtype));
+  if (nterms > size_t(kNoAllocMaxTerms))
+    return fail(QSV_EINVAL, "qsv_dev_expectation_noalloc takes at most " + std::to_string(kNoAllocMaxTerms) +
+                                " terms (got " + std::to_string(nterms) + ")");
+  std::vector<PauliOp> ops;
+  QSV_TRY(terms_to_ops(terms, nterms, n, ops));
+  void* scr = nullptr;
+  QSV_TRY(g_noalloc.get(&scr));
+  std::vector<double> v(nterms);
+  auto st = static_cast<hipStream_t>(stream);
+  if (nterms)
+    QSV_TRY(dtype == QSV_C128 ? expectation_core<double>(psi, n, ops, v.data(), scr, st)
+                              : expectation_core<float>(psi, n, ops, v.data(), scr, st));
   double s = 0;
   for (size_t i = 0; i < nterms; ++i) {
     if (values) values[i] = v[i];
