@@ -61,6 +61,10 @@ def parse():
                     help="time the std::vector entry (PCIe in and out) on the main workload (0 skips it)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the dominant kernels and a 16 GiB copy (for rocprofv3 --pmc)")
+    ap.add_argument("--trace-table", default="",
+                    help="print a markdown table of the kernels in a rocprofv3 results database "
+                         "(with --plain-json: the plain leg's launches joined to their algorithmic bytes)")
+    ap.add_argument("--plain-json", default="", help="the JSON line of a `--legs plain` run")
     ap.add_argument("--legs", default="",
                     help="comma list of side legs to run alone, without the main leg "
                          "(adjoint, adjoint_big, batch, precision; for profiling runs)")
@@ -214,6 +218,90 @@ def adjoint_big_leg(args):
             "max_abs_adjoint_minus_fd": float(np.abs(g_adj[which] - g_fd).max())}
 
 
+def plain_leg(args):
+    """The brickwork as plain per-gate sweeps (one launch per gate), run
+    twice, with every launch's gate and algorithmic bytes (S for a
+    controlled gate whose control bit lies above the 128-byte line, 2S
+    otherwise), so a kernel trace of this leg can be joined launch by launch
+    (the last run's dispatches) into per-kernel HBM rates."""
+    import torch
+
+    from qclabpp_amd import engine as E
+    from qclabpp_amd import gates as G
+    from qclabpp_amd import workloads as W
+
+    n = args.n
+    circ = W.brickwork(n, args.depth)
+    prog = E.Program(n, circ, fuse_k=0)
+    psi = E.new_state(n, torch.complex128)
+    prog.run(psi)
+    stream = torch.cuda.current_stream()
+    ms = timed(torch, lambda: prog.run(psi), 1, stream)
+    S = (1 << n) * 16
+    gates = []
+    for g in circ:
+        if g.kind == G._CX:
+            bc, bt = n - 1 - g.qubits[0], n - 1 - g.qubits[1]
+            alg = S if bc >= 3 else 2 * S
+            gates.append({"gate": g.name, "qubits": list(g.qubits), "control_bit": bc, "target_bit": bt,
+                          "alg_bytes": alg})
+        else:
+            gates.append({"gate": g.name, "qubits": list(g.qubits), "target_bit": n - 1 - g.qubits[0],
+                          "alg_bytes": 2 * S})
+    del psi
+    return {"workload": f"brickwork n={n} depth={args.depth}, complex128, one plain sweep per gate",
+            "ms_per_step": ms, "launches": len(circ), "alg_bytes": sum(x["alg_bytes"] for x in gates),
+            "hbm_GBps": sum(x["alg_bytes"] for x in gates) / (ms / 1e3) / 1e9, "gates": gates}
+
+
+def trace_table(db_path, plain_json=""):
+    """Markdown summary of a rocprofv3 results database (its `kernels` view):
+    dispatches, total and mean milliseconds per kernel.  With the JSON of a
+    `--legs plain` run, the last len(gates) dispatches are that run's
+    launches in gate order; each kernel then also gets its algorithmic bytes
+    and the HBM rate they imply, split by the gate's control bit position."""
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    rows = db.execute("select name, start, end from kernels order by start").fetchall()
+    short = lambda nm: nm.replace("HIP_vector_type<double, 2u>", "double2").replace(
+        "HIP_vector_type<float, 2u>", "float2").replace("qsv::(anonymous namespace)::", "").split("(")[0]
+    agg = {}
+    for nm, a, b in rows:
+        d = agg.setdefault(short(nm), [0, 0.0])
+        d[0] += 1
+        d[1] += (b - a) / 1e6
+    print("| kernel | dispatches | total ms | mean ms |")
+    print("|---|---:|---:|---:|")
+    for k, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{k}` | {c} | {t:.1f} | {t / c:.3f} |")
+    if not plain_json:
+        return 0
+    leg = json.loads(Path(plain_json).read_text())["plain"]
+    gates = leg["gates"]
+    run = rows[-len(gates):]
+    by = {}
+    for g, (nm, a, b) in zip(gates, run):
+        where = ""
+        if "control_bit" in g:
+            where = "control bit inside a line" if g["control_bit"] < 3 else "control bit above the line"
+        key = (short(nm), g["gate"] if g["gate"] == "CX" else "rotation", where)
+        d = by.setdefault(key, [0, 0.0, 0.0])
+        d[0] += 1
+        d[1] += (b - a) / 1e6
+        d[2] += g["alg_bytes"]
+    print()
+    print("| kernel | gates | launches | mean ms | algorithmic GB per launch | TB/s | of 8 TB/s |")
+    print("|---|---|---:|---:|---:|---:|---:|")
+    for (k, gk, where), (c, t, by_) in sorted(by.items(), key=lambda kv: -kv[1][1]):
+        rate = by_ / (t / 1e3) / 1e12
+        label = gk + (f", {where}" if where else "")
+        print(f"| `{k}` | {label} | {c} | {t / c:.3f} | {by_ / c / 1e9:.2f} | {rate:.2f} | {rate / 8:.2f} |")
+    tot = sum(v[1] for v in by.values())
+    print(f"\nAll {len(gates)} launches: {tot:.1f} ms, "
+          f"{sum(v[2] for v in by.values()) / (tot / 1e3) / 1e12:.2f} TB/s algorithmic.")
+    return 0
+
+
 def adjoint_roofline(prof, n, elem=16):
     """The adjoint's phases with the reverse sweep's roofline: each reverse
     launch reads and writes both states once, 4 * 2^n * 16 B algorithmic,
@@ -294,7 +382,12 @@ def batch_leg(args):
            "tuned": best, "tune_s": tune_s,
            "candidates": [{k: r[k] for k in ("tile_bits", "fuse_k", "ms")} for r in results],
            "max_abs_diff_between_candidates": max(r["max_abs_diff"] for r in results),
-           "passes": b.info["npasses"], "levels": b.info["nlevels"], "small_circuits": b.info["nsmall"]}
+           "passes": b.info["npasses"], "levels": b.info["nlevels"], "small_circuits": b.info["nsmall"],
+           "roofline": {"bound": "hbm", "achieved": b.info["alg_bytes"] / (ms / 1e3) / 1e9, "peak": PEAK_HBM,
+                        "unit": "GB/s", "frac": b.info["alg_bytes"] / (ms / 1e3) / 1e9 / PEAK_HBM,
+                        "alg_bytes_per_run": b.info["alg_bytes"],
+                        "note": "whole run (every launch of the graph): each tile pass reads and writes "
+                                "its circuit's state once, a first pass only writes it"}}
     if not args.no_cpu:
         import oracle
         sample = [i for i in range(args.batch)][:16]
@@ -476,11 +569,13 @@ def main():
         return 2
     if args.kernels_only:
         return kernels_only(args.n)
+    if args.trace_table:
+        return trace_table(args.trace_table, args.plain_json)
     import torch
 
     if args.legs:
         legs = {"adjoint": adjoint_leg, "adjoint_big": adjoint_big_leg, "batch": batch_leg,
-                "precision": precision_leg}
+                "precision": precision_leg, "plain": plain_leg}
         out = {}
         for name in args.legs.split(","):
             if name not in legs:

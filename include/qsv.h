@@ -322,6 +322,9 @@ typedef struct qsv_batch_info_t {
   int64_t  nlevels;    /* dependent launches of one run                     */
   uint64_t total_amplitudes;
   qsv_batch_opts opts; /* the options in effect                             */
+  double   alg_bytes;  /* algorithmic HBM bytes of one run: every pass reads
+                          and writes its circuit's state once (a circuit's
+                          first pass only writes it)                       */
 } qsv_batch_info_t;
 
 typedef struct qsv_tune_result {

@@ -61,7 +61,7 @@ class QsvBatchInfo(C.Structure):
 
     _fields_ = [("ncirc", C.c_int64), ("nsmall", C.c_int64), ("npasses", C.c_int64),
                 ("nops", C.c_int64), ("nlevels", C.c_int64), ("total_amplitudes", C.c_uint64),
-                ("opts", QsvBatchOpts)]
+                ("opts", QsvBatchOpts), ("alg_bytes", C.c_double)]
 
 
 class QsvTuneResult(C.Structure):

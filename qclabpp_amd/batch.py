@@ -57,7 +57,7 @@ class Batch:
                      "nops": inf.nops, "nlevels": inf.nlevels,
                      "total_amplitudes": inf.total_amplitudes,
                      "tile_bits": inf.opts.tile_bits, "small_bits": inf.opts.small_bits,
-                     "fuse_k": inf.opts.fuse_k}
+                     "fuse_k": inf.opts.fuse_k, "alg_bytes": inf.alg_bytes}
 
     def __del__(self):
         h = getattr(self, "_h", None)

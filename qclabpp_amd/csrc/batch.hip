@@ -333,6 +333,11 @@ int qsv_batch_info(const qsv_batch* b, qsv_batch_info_t* info, uint64_t* offsets
     int64_t s = 0;
     for (int c = 0; c < B.ncirc; ++c) s += B.nq[c] <= B.opts.small_bits;
     info->nsmall = s;
+    const double eb = B.dtype == QSV_C128 ? 16.0 : 8.0;
+    double bytes = 0;
+    for (const TPass& P : B.passes)
+      bytes += (P.init_zero ? 1.0 : 2.0) * std::ldexp(eb, B.nq[size_t(P.circuit)]);
+    info->alg_bytes = bytes;
   }
   return QSV_OK;
 }

@@ -152,11 +152,14 @@ __global__ __launch_bounds__(TPB) void k_cdense1(C* __restrict__ psi, uint64_t c
   }
 }
 
-// Controlled 2x2 with target bit < 6: shuffle across lanes, store only where
-// the control bit matches.
+// Controlled 2x2 with target bit < 6: shuffle across lanes; store where the
+// control bit matches, or every element (unchanged where it does not) when
+// `all` is set -- a control bit inside a 128-byte line, where storing only
+// the matching amplitudes would write every line in part.
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_cdense1_lanes(C* __restrict__ psi, uint64_t nel, int bt,
-                                                       int bc, int cs, M2<C> m, uint64_t blk0) {
+                                                       int bc, int cs, int all, M2<C> m,
+                                                       uint64_t blk0) {
   const int s = 1 << bt;
   const uint64_t e0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
   C x[NP];
@@ -172,7 +175,8 @@ __global__ __launch_bounds__(TPB) void k_cdense1_lanes(C* __restrict__ psi, uint
     const bool hi = (e >> bt) & 1;
     const C cself = hi ? m.a11 : m.a00;
     const C cother = hi ? m.a10 : m.a01;
-    if (e < nel && int((e >> bc) & 1) == cs) st_stream(&psi[e], cmad(cother, y, cmul(cself, x[j])));
+    const bool on = int((e >> bc) & 1) == cs;
+    if (e < nel && (on || all)) st_stream(&psi[e], on ? cmad(cother, y, cmul(cself, x[j])) : x[j]);
   }
 }
 
@@ -247,18 +251,27 @@ __global__ __launch_bounds__(TPB) void k_swap_lanes(C* __restrict__ psi, uint64_
 // ---- controlled 2x2, target bit below the lane count, control bit above it -----
 // Sweeps only the half with control bit == cs (thread k < N/2 owns element
 // ins0(k, bc) | cs<<bc); the target partner is lane ^ 2^bt.
-template <typename C>
+template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_cdense1_lanes_half(C* __restrict__ psi, uint64_t cnt,
                                                             int bt, int bc, int cs, M2<C> m,
                                                             uint64_t blk0) {
-  const uint64_t k = (blk0 + blockIdx.x) * TPB + threadIdx.x;
-  const uint64_t e = ins0(k < cnt ? k : 0, bc) | (uint64_t(cs) << bc);
-  const C x = ld_stream(&psi[e]);
-  const C y = shfl_xor_c(x, 1 << bt);
-  const bool hi = (e >> bt) & 1;
-  const C cself = hi ? m.a11 : m.a00;
-  const C cother = hi ? m.a10 : m.a01;
-  if (k < cnt) st_stream(&psi[e], cmad(cother, y, cmul(cself, x)));
+  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
+  C x[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    x[j] = ld_stream(&psi[ins0(k < cnt ? k : 0, bc) | (uint64_t(cs) << bc)]);
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint64_t k = k0 + uint64_t(j) * TPB;
+    const uint64_t e = ins0(k < cnt ? k : 0, bc) | (uint64_t(cs) << bc);
+    const C y = shfl_xor_c(x[j], 1 << bt);
+    const bool hi = (e >> bt) & 1;
+    const C cself = hi ? m.a11 : m.a00;
+    const C cother = hi ? m.a10 : m.a01;
+    if (k < cnt) st_stream(&psi[e], cmad(cother, y, cmul(cself, x[j])));
+  }
 }
 
 // ---- phase on a quarter whose lower bit lies inside a 128-byte line ----------
@@ -418,16 +431,23 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
     case K_CDENSE1: {
       const M2<C> m{to_c<C>(op.m[0]), to_c<C>(op.m[1]), to_c<C>(op.m[2]), to_c<C>(op.m[3])};
       const int bc = bit(op.q[0]), bt = bit(op.q[1]);
-      if (bt < tu.lane_bits && bc > bt && n >= 7) {
-        QSV_LAUNCH_BLOCKS((k_cdense1_lanes_half<C>), blocks_for(N / 2, TPB), TPB, 0, st, psi,
+      // The half sweep moves whole lines when the control bit lies outside
+      // them (n=30 complex128, brickwork CX: 2.76 ms, 0.78 of HBM peak for
+      // its S bytes).  With the control bit inside a line it would move
+      // 32-byte pieces of every line (6.97 ms); sweeping and storing all of
+      // the state instead takes 5.07 ms for the 2S counted for such a gate.
+      if (bt < tu.lane_bits && bc > bt && bc >= line_bits && n >= 7) {
+        QSV_LAUNCH_BLOCKS((k_cdense1_lanes_half<C, 1>), blocks_for(N / 2, TPB), TPB, 0, st, psi,
                           N / 2, bt, bc, op.cs, m);
       } else if (bt < tu.lane_bits && n >= 6) {
         QSV_LAUNCH_BLOCKS((k_cdense1_lanes<C, NP>), blocks_for(N, per), TPB, 0, st,
-                          psi, N, bt, bc, op.cs, m);
+                          psi, N, bt, bc, op.cs, int(bc < line_bits), m);
       } else if (bc < line_bits) {
         QSV_LAUNCH_BLOCKS((k_cdense1_full<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi, N / 2,
                           bt, bc, op.cs, m);
       } else {
+        // 1, 2 or 4 pairs per thread measure the same on the brickwork's CX
+        // gates (0.685-0.692 of HBM peak for their S bytes at n=30)
         const uint64_t a1 = uint64_t(op.cs) << bc;
         QSV_LAUNCH_BLOCKS((k_cdense1<C, NPC>), blocks_for(N / 4, TPB * NPC), TPB, 0, st,
                           psi, N / 4, std::min(bc, bt), std::max(bc, bt), a1,
