@@ -27,6 +27,7 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_HBM = 8000.0      # GB/s, MI355X HBM3E
 PEAK_FP64 = 78.6       # TFLOP/s dense FP64 matrix
+W_SEED_BRICK = 20261202  # seed of the brickwork workload (qclabpp_amd.workloads.SEED_BRICK)
 
 
 def parse():
@@ -47,7 +48,7 @@ def parse():
     ap.add_argument("--adjoint-layers", type=int, default=10)
     ap.add_argument("--adjoint-big-n", type=int, default=30,
                     help="qubits of the memory-scale adjoint run (three 2^n states; 0 skips it)")
-    ap.add_argument("--ps-params", type=int, default=6,
+    ap.add_argument("--ps-params", type=int, default=9,
                     help="parameters checked against the CPU parameter-shift gradient "
                          "(a seeded pick spread evenly over RY, RZ and RZZ)")
     ap.add_argument("--batch", type=int, default=512,
@@ -56,7 +57,9 @@ def parse():
     ap.add_argument("--precision-n", type=int, default=28,
                     help="qubits of the complex64-vs-complex128 comparison (0 skips it)")
     ap.add_argument("--precision-big-n", type=int, default=33,
-                    help="qubits of the complex64-only run (64 GiB at 33; 0 skips it)")
+                    help="qubits of the large complex64 run (64 GiB at 33; 0 skips it)")
+    ap.add_argument("--precision-big-c128", type=int, default=1,
+                    help="also run the large circuit in complex128 (128 GiB at 33) and compare")
     ap.add_argument("--host-entry", type=int, default=1,
                     help="time the std::vector entry (PCIe in and out) on the main workload (0 skips it)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -344,9 +347,26 @@ def host_entry_leg(args, circ, ms_jit):
     _lib.check(_lib.lib().qsv_release_cache())
     t = secs[-1]
     xfer = max(t - ms_jit / 1e3, 1e-9)
-    return {"workload": f"brickwork n={n} depth={args.depth}, complex128, host vector in and out",
-            "s": t, "first_call_s": secs[0], "gate_passes_per_s_incl_pcie": len(circ) / t,
-            "transfer_s": xfer, "pcie_GBps_both_ways": 2 * v.nbytes / xfer / 1e9}
+    del v
+    out = {"workload": f"brickwork n={n} depth={args.depth}, complex128, host vector in and out",
+           "s": t, "first_call_s": secs[0], "gate_passes_per_s_incl_pcie": len(circ) / t,
+           "transfer_s": xfer, "pcie_GBps_both_ways": 2 * (1 << n) * 16 / xfer / 1e9}
+    # the same through the C++ drop-in: QCircuit<std::complex<double>> built
+    # from RotationX/Y/Z and CNOT objects, simulate(std::vector&) with the
+    # library's default form (compiled tiled passes from 22 qubits up), in a
+    # process of its own
+    import subprocess
+    exe = ROOT / "tests" / "cpp" / "_bin" / "dropin_test"
+    if exe.exists():
+        res = subprocess.run([str(exe), "--time-brickwork", str(n), str(args.depth), str(W_SEED_BRICK)],
+                             capture_output=True, text=True, timeout=600)
+        if res.returncode == 0:
+            cpp = json.loads(res.stdout.strip().splitlines()[-1])
+            cpp["gate_passes_per_s_incl_pcie"] = cpp["gates"] / cpp["s"]
+            out["cpp_api"] = cpp
+        else:
+            out["cpp_api"] = {"error": (res.stdout + res.stderr)[-1000:]}
+    return out
 
 
 def batch_leg(args):
@@ -408,9 +428,10 @@ def precision_leg(args):
     20261203), both run with the compiled tiled passes from |0...0>.  At n=28 the
     complex64 state is compared with the complex128 one: max |delta amp|,
     norm drift |<psi|psi> - 1| and the error of <Z_i Z_{i+1}> (i = 0..n-2)
-    and of their sum, every reduction accumulated in FP64.  At n=33 (64 GiB)
-    only complex64 fits; its FP64 norm drift, <Z_i Z_{i+1}> and run time are
-    reported."""
+    and of their sum, every reduction accumulated in FP64.  At n=33 the
+    complex64 state (64 GiB) is run, and the complex128 one (128 GiB) beside
+    it on the same GPU: the same comparisons, max |delta amp| taken over
+    2^27-amplitude chunks."""
     import numpy as np
     import torch
 
@@ -458,6 +479,22 @@ def precision_leg(args):
             "hbm_GBps": info["alg_bytes"] / (ms / 1e3) / 1e9, "launches": info["nlaunch"],
             "norm_drift_fp64": abs(O.norm2(psi) - 1.0),
             "zz": [float(v) for v in vals], "zz_sum": tot}
+        del prog
+        if args.precision_big_c128:
+            p128 = E.Program(n, circ, fuse_k=E.TILED_JIT, dtype="c128")
+            ref = E.new_state(n, torch.complex128)
+            ms128 = timed(torch, lambda: p128.run(ref), 1, stream)
+            v128, t128 = O.expectation(ref, O.zz_chain(n))
+            chunk = 1 << 27
+            dmax = 0.0
+            for a in range(0, 1 << n, chunk):
+                dmax = max(dmax, (psi[a:a + chunk].to(torch.complex128) - ref[a:a + chunk]).abs().max().item())
+            out["n33"].update({
+                "ms_c128": ms128, "norm_drift_c128": abs(O.norm2(ref) - 1.0),
+                "max_abs_amp_c64_vs_c128": dmax,
+                "max_abs_zz_err_c64": float(np.abs(np.asarray(vals) - np.asarray(v128)).max()),
+                "zz_sum_c128": t128, "abs_zz_sum_err_c64": abs(tot - t128)})
+            del ref, p128
         del psi
         torch.cuda.empty_cache()
     return out

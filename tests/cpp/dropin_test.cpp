@@ -14,8 +14,13 @@
 //   dropin_test --qasm-rewrite FILE
 //                        FILE read in the extended dialect and written back
 //                        with QCircuit::toQASM (no GPU)
+//   dropin_test --time-brickwork N DEPTH SEED
+//                        QCircuit<complex<double>>::simulate of the seeded
+//                        brickwork on a std::vector from |0...0>, twice,
+//                        with the library's default form; prints JSON
 #include <hip/hip_runtime_api.h>
 
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -533,7 +538,58 @@ int qasm_rewrite(const char* path) {
   return 0;
 }
 
+// The brickwork of the benchmark built through the drop-in classes: per
+// layer a rotation on every qubit (axis then angle from SplitMix64, as
+// qclabpp_amd/workloads.py draws them), then CNOT(i, i+1) for i of the
+// layer's parity.
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return double(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+int time_brickwork(int n, int depth, uint64_t seed) {
+  using T = std::complex<double>;
+  SplitMix64 rng{seed};
+  QCircuit<T> c(n);
+  for (int layer = 0; layer < depth; ++layer) {
+    for (int q = 0; q < n; ++q) {
+      const int axis = std::min(int(rng.uniform() * 3), 2);
+      const double th = rng.uniform() * 2 * M_PI;
+      if (axis == 0) c.push_back(std::make_unique<RotationX<T>>(q, th));
+      else if (axis == 1) c.push_back(std::make_unique<RotationY<T>>(q, th));
+      else c.push_back(std::make_unique<RotationZ<T>>(q, th));
+    }
+    for (int i = layer % 2; i + 1 < n; i += 2) c.push_back(std::make_unique<CNOT<T>>(i, i + 1));
+  }
+  std::vector<T> v(size_t(1) << n);
+  double secs[2];
+  for (int rep = 0; rep < 2; ++rep) {
+    std::fill(v.begin(), v.end(), T(0));
+    v[0] = 1;
+    const auto t0 = std::chrono::steady_clock::now();
+    c.simulate(v);
+    secs[rep] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  double norm = 0;
+  for (const T& x : v) norm += std::norm(x);
+  qsv_cache_stats st{};
+  (void)qsv_cache_stats_get(&st);
+  std::printf("{\"n\": %d, \"gates\": %zu, \"first_call_s\": %.6f, \"s\": %.6f, \"norm\": %.15f, "
+              "\"schedule_hits\": %lld, \"module_compiles\": %lld, \"module_hits\": %lld}\n",
+              n, c.nbGates(), secs[0], secs[1], norm, (long long)st.schedule_hits,
+              (long long)st.module_compiles, (long long)st.module_hits);
+  return std::abs(norm - 1.0) < 1e-9 ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 4 && std::strcmp(argv[1], "--time-brickwork") == 0)
+    return time_brickwork(std::atoi(argv[2]), std::atoi(argv[3]), std::strtoull(argv[4], nullptr, 10));
   if (argc > 3 && std::strcmp(argv[1], "--qasm-dump") == 0) return qasm_dump(argv[2], argv[3]);
   if (argc > 2 && std::strcmp(argv[1], "--qasm-rewrite") == 0) return qasm_rewrite(argv[2]);
   const bool host_only = argc > 1 && std::strcmp(argv[1], "--host") == 0;

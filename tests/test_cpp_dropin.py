@@ -55,3 +55,19 @@ def test_dropin_on_gpu(tmp_path):
     print(res.stdout[-4000:])
     assert res.returncode == 0, res.stdout[-4000:] + res.stderr[-2000:]
     assert "0 failed" in res.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_brickwork_timing_mode(tmp_path):
+    """The C++ timing mode the benchmark uses: a seeded brickwork built from
+    the drop-in gate classes, simulated twice on a std::vector; the second
+    call reuses the schedule and compiled passes of the first."""
+    import json
+    import os
+    env = dict(os.environ, QSV_JIT_CACHE_DIR=str(tmp_path))
+    res = subprocess.run([str(_binary()), "--time-brickwork", "22", "4", "20261202"], capture_output=True,
+                         text=True, timeout=300, env=env)
+    assert res.returncode == 0, res.stdout + res.stderr
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    assert out["gates"] == 4 * 22 + 11 + 10 and abs(out["norm"] - 1) < 1e-9
+    assert out["schedule_hits"] >= 1 and out["module_hits"] >= 1
