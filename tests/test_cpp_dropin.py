@@ -69,5 +69,5 @@ def test_cpp_brickwork_timing_mode(tmp_path):
                          text=True, timeout=300, env=env)
     assert res.returncode == 0, res.stdout + res.stderr
     out = json.loads(res.stdout.strip().splitlines()[-1])
-    assert out["gates"] == 4 * 22 + 11 + 10 and abs(out["norm"] - 1) < 1e-9
+    assert out["gates"] == 4 * 22 + 2 * 11 + 2 * 10 and abs(out["norm"] - 1) < 1e-9
     assert out["schedule_hits"] >= 1 and out["module_hits"] >= 1
