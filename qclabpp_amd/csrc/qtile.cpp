@@ -736,14 +736,15 @@ void qtile_scale_rotations(QProgram& qp) {
   constexpr int NR = 1 << kQRegBits;
   // A scaled rotation takes out a factor of modulus in [2^-1/2, 1] and a
   // unitary diagonal one of modulus 1, so after kMaxScaled scaled rotations
-  // the product is still at least 2^-20: the stored amplitudes stay within
-  // 2^20 of their true scale however many gates a pass holds (a deep
+  // the product is still at least 2^-60: the stored amplitudes stay within
+  // 2^60 of their true scale (far inside the FP32 and FP64 exponent ranges;
+  // scaling costs no relative precision) however many gates a pass holds (a deep
   // circuit on a small state is one pass).  The next rotation then runs as
   // a complex 2x2 times the product, which resets it.  The reset point is a
   // count, not a test of the product, so the ops emitted (and the code
   // compiled from them) depend on the circuit's structure, never on its
   // angles.
-  constexpr int kMaxScaled = 40;
+  constexpr int kMaxScaled = 120;
   std::vector<QOp> ops;
   ops.reserve(qp.ops.size() + qp.passes.size());
   for (const QPass& P : qp.passes) {
@@ -762,7 +763,7 @@ void qtile_scale_rotations(QProgram& qp) {
             unit && (x.kind == Q_ROTP || (x.kind == Q_XLIKEP && x.m[0] == x.m[6] && x.m[3] == x.m[5]));
         if (scalable && nscaled == kMaxScaled) {
           // the factors taken out so far may have shrunk the stored
-          // amplitudes' scale by up to 2^20: this op, times lam, becomes a
+          // amplitudes' scale by up to 2^60: this op, times lam, becomes a
           // complex 2x2 that puts them back at their true scale
           const cd e[4] = {cd(x.m[0]),
                            x.kind == Q_ROTP ? cd(x.m[2]) : cd(0.0, x.m[3]),

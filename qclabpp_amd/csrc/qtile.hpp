@@ -120,9 +120,9 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
 // is max(|c|, |s|) times a matrix with a unit entry pair), factors the most
 // common entry out of each diagonal op so its unit entries drop out, and
 // folds the product of the factors into a diagonal op at the end of the
-// pass.  Each factor is at least 1/sqrt(2) in modulus; after 40 of them the
+// pass.  Each factor is at least 1/sqrt(2) in modulus; after 120 of them the
 // next rotation is kept as a complex 2x2 times their product, which resets
-// it, so the stored amplitudes stay within 2^20 of their true scale in
+// it, so the stored amplitudes stay within 2^60 of their true scale in
 // passes of any length.  The reset points depend on the structure only.
 void qtile_scale_rotations(QProgram& qp);
 

@@ -73,11 +73,11 @@ def test_tiled_every_kind_c64(n, form):
 def test_tiled_deep_pass_keeps_scale(dt, form, count):
     """Thousands of rotations on a 12-qubit state are one pass (its tile
     is the whole state), interpreted and compiled.  The rotations run in
-    scaled forms whose factors multiply up over the pass; every 40 scaled
-    rotations one runs as a complex 2x2 that resets the product, so the
-    stored amplitudes stay within 2^20 of their true scale and neither
+    scaled forms whose factors multiply up over the pass; after 120 scaled
+    rotations the next runs as a complex 2x2 that resets the product, so the
+    stored amplitudes stay within 2^60 of their true scale and neither
     precision overflows.  The compiled pass is kept to 600 rotations
-    (15 resets) so that its straight-line code compiles in seconds."""
+    (4 resets) so that its straight-line code compiles in seconds."""
     print(f"tiled deep pass {dt} form={form} rotations={count}", flush=True)
     n = 12
     rs = np.random.default_rng(31)

@@ -70,8 +70,8 @@ def test_generic_angles_share_compiled_code(dtype):
 
 @pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
 def test_deep_pass_source_is_angle_independent(dtype):
-    """A deep circuit on 12 qubits is one pass whose scaled rotations reset
-    their factored-out product many times.  The reset points depend on the
+    """A deep circuit on 12 qubits (360 rotations) is one pass whose scaled
+    rotations reset their factored-out product twice.  The reset points depend on the
     structure only, so every angle set -- small and large angles, cos or sin
     the larger, angles that take the product to its bound fastest -- gives
     the same generated source and shares one compiled module."""
@@ -81,7 +81,7 @@ def test_deep_pass_source_is_angle_independent(dtype):
 
     def circ(angle):
         out = []
-        for layer in range(14):
+        for layer in range(30):
             for q in range(n):
                 out.append((G.RotationX, G.RotationY)[(q + layer) % 2](q, angle(layer, q)))
             for i in range(layer % 2, n - 1, 2):
