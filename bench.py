@@ -68,6 +68,10 @@ def parse():
                     help="print a markdown table of the kernels in a rocprofv3 results database "
                          "(with --plain-json: the plain leg's launches joined to their algorithmic bytes)")
     ap.add_argument("--plain-json", default="", help="the JSON line of a `--legs plain` run")
+    ap.add_argument("--launches", default="",
+                    help="with --trace-table: KERNEL:N:FETCH_DB:WRITE_DB -- the last N dispatches of "
+                         "KERNEL, one row each, with the bytes the two counter runs' last N dispatches "
+                         "of it read (FETCH_SIZE x2.0) and wrote (WRITE_SIZE x1.0)")
     ap.add_argument("--legs", default="",
                     help="comma list of side legs to run alone, without the main leg "
                          "(adjoint, adjoint_big, batch, precision; for profiling runs)")
@@ -268,6 +272,19 @@ def trace_table(db_path, plain_json=""):
     rows = db.execute("select name, start, end from kernels order by start").fetchall()
     short = lambda nm: nm.replace("HIP_vector_type<double, 2u>", "double2").replace(
         "HIP_vector_type<float, 2u>", "float2").replace("qsv::(anonymous namespace)::", "").split("(")[0]
+    counters = db.execute("select kernel_name, counter_name, avg(value), count(value) from counters_collection "
+                          "group by kernel_name, counter_name").fetchall()
+    if counters:
+        # a counter run: per-kernel means of each counter per dispatch
+        names = sorted({c for _, c, _, _ in counters})
+        table = {}
+        for k, c, v, cnt in counters:
+            table.setdefault(short(k), {})[c] = v
+        print("| kernel | " + " | ".join(names) + " |")
+        print("|---|" + "---:|" * len(names))
+        for k in sorted(table, key=lambda x: (len(x), x)):
+            print(f"| `{k}` | " + " | ".join(f"{table[k].get(c, float('nan')):.4g}" for c in names) + " |")
+        return 0
     agg = {}
     for nm, a, b in rows:
         d = agg.setdefault(short(nm), [0, 0.0])
@@ -302,6 +319,44 @@ def trace_table(db_path, plain_json=""):
     tot = sum(v[1] for v in by.values())
     print(f"\nAll {len(gates)} launches: {tot:.1f} ms, "
           f"{sum(v[2] for v in by.values()) / (tot / 1e3) / 1e12:.2f} TB/s algorithmic.")
+    return 0
+
+
+def launch_table(db_path, spec):
+    """The last N dispatches of one kernel in a kernel-trace database, with
+    start and duration, joined in order to the last N dispatches of the same
+    kernel in a FETCH_SIZE and a WRITE_SIZE counter database (KiB per
+    dispatch; FETCH_SIZE scaled x2.0 and WRITE_SIZE x1.0, what a 16 GiB copy
+    gives on the MI355X)."""
+    import sqlite3
+    kernel, count, fdb, wdb = spec.split(":")
+    count = int(count)
+    rows = [r for r in sqlite3.connect(db_path).execute("select name, start, end from kernels order by start")
+            if kernel in r[0]][-count:]
+
+    def per_dispatch(path):
+        vals = {}
+        for d, k, v in sqlite3.connect(path).execute(
+                "select dispatch_id, kernel_name, value from counters_collection order by dispatch_id"):
+            if kernel in k:
+                vals[d] = vals.get(d, 0.0) + v
+        return [vals[d] for d in sorted(vals)][-count:]
+
+    fetch, write = per_dispatch(fdb), per_dispatch(wdb)
+    t0 = rows[0][1]
+    print("| launch | start ms | ms | read GB | written GB | TB/s |")
+    print("|---:|---:|---:|---:|---:|---:|")
+    tot_ms = tot_b = 0.0
+    for i, ((_, a, b), f, w) in enumerate(zip(rows, fetch, write)):
+        rd, wr = f * 2.0 * 1024, w * 1024
+        ms = (b - a) / 1e6
+        tot_ms += ms
+        tot_b += rd + wr
+        print(f"| {i} | {(a - t0) / 1e6:.2f} | {ms:.3f} | {rd / 1e9:.2f} | {wr / 1e9:.2f} | "
+              f"{(rd + wr) / (ms / 1e3) / 1e12:.2f} |")
+    print(f"\n{len(rows)} launches: {tot_ms:.1f} ms of kernels, {tot_b / 1e9:.2f} GB, "
+          f"{tot_b / (tot_ms / 1e3) / 1e12:.2f} TB/s; span from the first start to the last end "
+          f"{(rows[-1][2] - rows[0][1]) / 1e6:.1f} ms.")
     return 0
 
 
@@ -606,6 +661,8 @@ def main():
         return 2
     if args.kernels_only:
         return kernels_only(args.n)
+    if args.trace_table and args.launches:
+        return launch_table(args.trace_table, args.launches)
     if args.trace_table:
         return trace_table(args.trace_table, args.plain_json)
     import torch
