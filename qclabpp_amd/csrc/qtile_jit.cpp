@@ -476,6 +476,315 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
     << "}\n";
 }
 
+// ---- passes through a half-tile LDS buffer ----------------------------------
+//
+// A phase change moves the tile through LDS in two rounds split on one
+// tile-local bit b: round h moves the amplitudes whose position has bit b
+// equal to h.  The buffer then holds half a tile (2^(T-1) amplitudes plus
+// padding: 33 KiB for a complex128 tile of 2^12), so LDS no longer limits a
+// CU to two blocks and the register budget sets occupancy: three waves per
+// SIMD, no register prefetch (the other blocks hide the load latency).
+//
+// Slots: the position with bit b dropped and the top tile bit moved into its
+// place ("compressed", T-1 bits), padded as q + (q >> 5) + (q >> 10).  Both
+// maps are additive over disjoint bits, so a register's slot is the thread's
+// base slot plus a literal (one address register per side, immediate
+// offsets).  Bank rules measured on gfx950 with a probe kernel: ds_read_b128
+// serves 32 lanes at a time and is conflict-free when their 16-byte slots
+// differ mod 32; ds_write_b128 serves 16 lanes, slots distinct mod 16.  With
+// this padding a compressed bit m moves the slot by 2^(m mod 5) mod 32 (its
+// "class"), so reads are conflict-free when lane bits 0-4 hold the five
+// classes, and writes when lane bits 0-3 hold classes 0-3.
+
+// One side of a phase change: thread bit j -> local bit ord[j]; register i
+// at local position offset off[i] (register bits only).
+struct XSide {
+  std::vector<int> ord;
+  uint32_t off[NR];
+  uint32_t regmask;
+};
+
+// Local bit m after compression by the split bit b (m != b); b < 0: none.
+int xbit(int m, int b, int T) { return b >= 0 && m == T - 1 && b != T - 1 ? b : m; }
+
+uint32_t xcompress(uint32_t p, int b, int T) {
+  uint32_t q = 0;
+  for (int m = 0; m < T; ++m)
+    if (m != b && (p >> m & 1u)) q |= 1u << xbit(m, b, T);
+  return q;
+}
+
+uint32_t xpad(uint32_t q) { return q + (q >> 5) + (q >> 10); }
+
+int xclass(int m, int b, int T) { return xbit(m, b, T) % 5; }
+
+// Conflict-free lanes of an order under split bit b: reads count the lane
+// bits 0-4 of distinct classes (5 = conflict-free), writes whether lane
+// bits 0-3 hold classes 0-3.
+int read_lanes_ok(const std::vector<int>& ord, int b, int T) {
+  unsigned seen = 0;
+  int n = 0;
+  for (size_t j = 0; j < 5 && j < ord.size(); ++j) {
+    const int c = xclass(ord[j], b, T);
+    if (!(seen >> c & 1u)) ++n;
+    seen |= 1u << c;
+  }
+  return n;
+}
+bool write_lanes_ok(const std::vector<int>& ord, int b, int T) {
+  unsigned seen = 0;
+  for (size_t j = 0; j < 4 && j < ord.size(); ++j) {
+    if (ord[j] == b) continue;  // that lane bit sits out each round in turn
+    const int c = xclass(ord[j], b, T);
+    if (c == 4 || (seen >> c & 1u)) return false;
+    seen |= 1u << c;
+  }
+  return true;
+}
+
+// Thread order of a phase of the split kernels: as lane_order, with the
+// classes of the padded compressed slots (split bit b of the phase change
+// that fills the registers; b < 0 for a phase loaded from HBM): lane bits
+// 0-3 take classes 0-3, lane bit 4 class 4, after the lines a direct first
+// phase loads; the lines a direct last phase stores stay among the 6 lane
+// bits.
+std::vector<int> lane_order_split(int T, uint32_t regmask, uint32_t first, uint32_t lanes, int b) {
+  std::vector<int> ord;
+  auto free_bit = [&](int m) {
+    return !(regmask >> m & 1u) && std::find(ord.begin(), ord.end(), m) == ord.end();
+  };
+  auto used = [&](int c) {
+    for (size_t i = 0; i < ord.size() && i < 5; ++i)
+      if (xclass(ord[i], b, T) == c) return true;
+    return false;
+  };
+  for (int m = 0; m < T; ++m)
+    if ((first >> m & 1u) && free_bit(m)) ord.push_back(m);
+  while (ord.size() < 5) {
+    auto wanted = [&](int m) {
+      const int c = xclass(m, b, T);
+      return !used(c) && (ord.size() < 4 ? c < 4 : c == 4);
+    };
+    int pick = -1;
+    for (int pass = 0; pass < 4 && pick < 0; ++pass)
+      for (int m = 0; m < T && pick < 0; ++m) {
+        if (!free_bit(m)) continue;
+        if ((pass == 0 && wanted(m) && (lanes >> m & 1u)) || (pass == 1 && wanted(m)) ||
+            (pass == 2 && !used(xclass(m, b, T))) || pass == 3)
+          pick = m;
+      }
+    if (pick < 0) break;
+    ord.push_back(pick);
+  }
+  for (int m = 0; m < T; ++m)
+    if ((lanes >> m & 1u) && free_bit(m)) ord.push_back(m);
+  for (int m = 0; m < T; ++m)
+    if (free_bit(m)) ord.push_back(m);
+  return ord;
+}
+
+// The exchange itself: registers v (side s) -> registers v (side d).
+void emit_exchange(std::ostringstream& s, const XSide& src, const XSide& dst, int b, int T) {
+  auto base = [&](const XSide& x, const char* name) {
+    s << "      const unsigned " << name << "q = 0u";
+    for (size_t j = 0; j < x.ord.size(); ++j)
+      if (x.ord[j] != b) s << " | (((y >> " << j << ") & 1u) << " << xbit(x.ord[j], b, T) << ")";
+    s << ";\n      const unsigned " << name << " = " << name << "q + (" << name << "q >> 5) + (" << name
+      << "q >> 10);\n";
+  };
+  auto thread_bit = [&](const XSide& x) {
+    for (size_t j = 0; j < x.ord.size(); ++j)
+      if (x.ord[j] == b) return int(j);
+    return -1;
+  };
+  s << "    {  // phase change through the half buffer, split on local bit " << b << "\n"
+    << "      unsigned y = tid;\n"
+    << "      asm volatile(\"\" : \"+v\"(y));\n";
+  base(src, "as");
+  base(dst, "ad");
+  const int js = (src.regmask >> b & 1u) ? -1 : thread_bit(src);
+  for (int h = 0; h < 2; ++h) {
+    s << "      __syncthreads();\n";
+    if (js >= 0) s << "      if (((y >> " << js << ") & 1u) == " << h << "u) {\n";
+    for (int i = 0; i < NR; ++i)
+      if (js >= 0 || int(src.off[i] >> b & 1u) == h)
+        s << "      tile[as + " << xpad(xcompress(src.off[i], b, T)) << "u] = v[" << i << "];\n";
+    if (js >= 0) s << "      }\n";
+    s << "      __syncthreads();\n";
+    for (int i = 0; i < NR; ++i)
+      if (int(dst.off[i] >> b & 1u) == h)
+        s << "      const C u" << i << " = tile[ad + " << xpad(xcompress(dst.off[i], b, T)) << "u];\n";
+  }
+  for (int i = 0; i < NR; ++i) s << "      v[" << i << "] = u" << i << ";\n";
+  s << "    }\n";
+}
+
+// A pass as one kernel with half-buffer phase changes.  Each tile: load the
+// registers (the first phase's, or tile positions tid + threads*j), phase
+// changes and ops, store the registers (the last phase's, or positions
+// tid + threads*j).  Returns false where a phase's register offsets leave
+// its register bits (never for the schedules built here); the caller then
+// emits the full-tile kernel.
+bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) {
+  const QPass& P = qp.passes[pi];
+  const int T = qp.tile_bits;
+  const unsigned threads = 1u << (T - kQRegBits);
+  const uint32_t tile_order = ((1u << kQRegBits) - 1) << (T - kQRegBits);
+  const int K = P.phase_end - P.phase_begin;
+  const QPhase& F = qp.phases[P.phase_begin];
+  const QPhase& L = qp.phases[P.phase_end - 1];
+  const uint32_t line_in = line_mask(P.tbits, T, c128), line_out = line_mask(P.tbits_out, T, c128);
+  const bool din = !(F.regmask & line_in);
+  const bool relabels = std::memcmp(P.tbits, P.tbits_out, sizeof(P.tbits)) != 0;
+  // a relabeling one-phase pass with both ends direct would have no barrier
+  // between the loads and the stores of its block; it stores through LDS
+  bool dout = !(L.regmask & line_out) && !(K == 1 && din && relabels);
+  // a direct store needs every written line bit among the 6 lane bits
+  // (always the case for the schedules built here; else store through LDS)
+  XSide tile_side;  // positions tid + threads*j
+  tile_side.ord = lane_order_split(T, tile_order, 0u, 0u, -1);
+  tile_side.regmask = tile_order;
+  for (int i = 0; i < NR; ++i) tile_side.off[i] = threads * uint32_t(i);
+  // each phase at its gather and at its write-back
+  std::vector<XSide> gat(K), sct(K);
+  for (int k = 0; k < K; ++k) {
+    const QPhase& Q = qp.phases[P.phase_begin + k];
+    gat[k].regmask = sct[k].regmask = Q.regmask;
+    for (int i = 0; i < NR; ++i) {
+      gat[k].off[i] = qtile_swizzle(Q.regpos[i]);
+      sct[k].off[i] = qtile_swizzle(Q.regpos_out[i]);
+      if ((gat[k].off[i] | sct[k].off[i]) & ~Q.regmask) return false;
+    }
+  }
+  // split bits (bin[k]: the phase change into phase k, bin[K]: into the
+  // store layout) chosen with the thread order of the phase they fill: a
+  // register bit of the receiving phase (every thread reads 8 amplitudes per
+  // round, with no lane conditions: the compiler would otherwise merge and
+  // spill the undefined halves of the register array), of the sending phase
+  // too where possible (every thread also writes 8); then the reads and the
+  // writes free of bank conflicts; then the highest bit.
+  std::vector<int> bin(K + 1, -1);
+  auto choose = [&](const XSide& src, uint32_t dmask, uint32_t lanes, std::vector<int>& ord_out) {
+    int best = -1;
+    long best_score = -1;
+    for (int b = T - 1; b >= 0; --b) {
+      if (!(dmask >> b & 1u)) continue;
+      std::vector<int> ord = dmask == tile_order ? tile_side.ord : lane_order_split(T, dmask, 0u, lanes, b);
+      const long score = ((src.regmask >> b & 1u) ? 100 : 0) + 10 * read_lanes_ok(ord, b, T) +
+                         (write_lanes_ok(src.ord, b, T) ? 1 : 0);
+      if (score > best_score) {
+        best_score = score;
+        best = b;
+        ord_out = ord;
+      }
+    }
+    return best;
+  };
+  auto build = [&]() {
+    for (int k = 0; k < K; ++k) {
+      const QPhase& Q = qp.phases[P.phase_begin + k];
+      const uint32_t lanes = k == K - 1 && dout ? line_out : 0u;
+      if (k == 0 && din) {
+        gat[0].ord = lane_order_split(T, Q.regmask, line_in, lanes, -1);
+      } else {
+        const XSide& src = k == 0 ? tile_side : sct[k - 1];
+        bin[k] = choose(src, Q.regmask, lanes, gat[k].ord);
+      }
+      sct[k].ord = gat[k].ord;
+    }
+    if (!dout) {
+      std::vector<int> ord;
+      bin[K] = choose(sct[K - 1], tile_order, 0u, ord);
+    }
+  };
+  build();
+  if (dout) {
+    for (int j = kLaneBitsJit; j < int(sct[K - 1].ord.size()); ++j)
+      if (line_out >> sct[K - 1].ord[j] & 1u) dout = false;
+    if (!dout) build();
+  }
+  const XSide& first = din ? gat[0] : tile_side;
+  const XSide& last = dout ? sct[K - 1] : tile_side;
+  std::vector<uint64_t> in_ofs(NR), out_ofs(NR);
+  for (int i = 0; i < NR; ++i) {
+    in_ofs[i] = gofs(P.tbits, T, first.off[i]);
+    out_ofs[i] = gofs(P.tbits_out, T, last.off[i]);
+  }
+  const bool lds = !din || !dout || K > 1;
+  s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(3)))\n"
+    << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
+  if (lds) s << "  __shared__ C tile[" << xpad((1u << (T - 1)) - 1) + 1 << "];\n";
+  s << "  const unsigned tid = threadIdx.x;\n"
+    << "  auto tile_base = [&](unsigned long long t) {\n"
+    << "    unsigned long long b = 0;\n";
+  for (int r = 0, sh = 0; r < P.nruns; ++r) {
+    s << "    b |= ((t >> " << sh << ") & " << u64((uint64_t(1) << P.rlen[r]) - 1) << ") << " << int(P.rdst[r])
+      << ";\n";
+    sh += P.rlen[r];
+  }
+  s << "    return b;\n"
+    << "  };\n"
+    << "  const unsigned long long gin = " << thread_gofs(P.tbits, first.ord) << ";\n"
+    << "  const unsigned long long gout = " << thread_gofs(P.tbits_out, last.ord) << ";\n";
+  auto arr = [&](const char* name, const std::vector<uint64_t>& v) {
+    s << "  const unsigned long long " << name << "[16] = {";
+    for (int i = 0; i < NR; ++i) s << (i ? ", " : "") << u64(v[i]);
+    s << "};\n";
+  };
+  arr("oin", in_ofs);
+  arr("oout", out_ofs);
+  // each block walks a contiguous range of tiles
+  s << "  const unsigned long long per = (ntiles + gridDim.x - 1) / gridDim.x;\n"
+    << "  unsigned long long t = blockIdx.x * per;\n"
+    << "  const unsigned long long tend = t + per < ntiles ? t + per : ntiles;\n"
+    << "  for (; t < tend; ++t) {\n"
+    << "    const unsigned long long base = tile_base(t);\n"
+    << "    C v[16];\n"
+    // the thread's offsets pass through empty asm statements inside the
+    // loop, so the 16 load and 16 store addresses are formed per tile
+    // instead of being hoisted out of the loop (and spilled)
+    << "    {\n"
+    << "      unsigned long long g = gin;\n"
+    << "      asm volatile(\"\" : \"+v\"(g));\n"
+    << "      const unsigned long long b0 = base + g;\n"
+    << "#pragma unroll\n"
+    << "      for (int j = 0; j < 16; ++j) v[j] = ldnt(&psi[b0 + oin[j]]);\n"
+    << "    }\n";
+  if (!din) emit_exchange(s, tile_side, gat[0], bin[0], T);
+  for (int k = 0; k < K; ++k) {
+    const int ph = P.phase_begin + k;
+    const QPhase& Q = qp.phases[ph];
+    // the coefficient pointer passes through an empty asm statement at each
+    // phase, so coefficient loads are formed in the phase that uses them
+    // instead of being hoisted out of the tile loop
+    s << "    {  // phase " << ph << "\n"
+      << "      const QC R* cf = cf0;\n"
+      << "      asm volatile(\"\" : \"+s\"(cf));\n";
+    for (int o = Q.op_begin; o < Q.op_end; ++o) emit_op(s, qp.ops[o], size_t(o), c128);
+    s << "    }\n";
+    if (k + 1 < K) emit_exchange(s, sct[k], gat[k + 1], bin[k + 1], T);
+  }
+  if (!dout) emit_exchange(s, sct[K - 1], tile_side, bin[K], T);
+  s << "    {\n"
+    << "      unsigned long long g = gout;\n"
+    << "      asm volatile(\"\" : \"+v\"(g));\n"
+    << "      const unsigned long long bo = base + g;\n"
+    << "#pragma unroll\n"
+    << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], v[j]);\n"
+    << "    }\n"
+    << "  }\n"
+    << "}\n";
+  return true;
+}
+
+// Complex128 passes of 2^12-amplitude tiles use the half buffer unless
+// QSV_JIT_SPLIT=0 (full-tile kernels, two waves per SIMD: for comparison).
+bool use_split(const QProgram& qp, bool c128) {
+  if (!c128 || qp.tile_bits != kQTileBits) return false;
+  const char* e = std::getenv("QSV_JIT_SPLIT");
+  return !(e && e[0] == '0');
+}
+
 // ---- adjoint reverse sweep: passes over two states -------------------------
 //
 // The same tile walk and phases as emit_pass, over phi (psi, registers v)
@@ -970,7 +1279,15 @@ void qtile_jit_cache_stats(int64_t* mem_hits, int64_t* disk_hits, int64_t* compi
 std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1) {
   std::ostringstream s;
   s << "#define REAL " << (c128 ? "double" : "float") << "\n" << kPrelude;
-  for (size_t i = p0; i < p1 && i < qp.passes.size(); ++i) emit_pass(s, qp, i, c128);
+  const bool split = use_split(qp, c128);
+  for (size_t i = p0; i < p1 && i < qp.passes.size(); ++i) {
+    std::ostringstream k;
+    if (split && emit_pass_split(k, qp, i, c128)) {
+      s << k.str();
+      continue;
+    }
+    emit_pass(s, qp, i, c128);
+  }
   return s.str();
 }
 
