@@ -38,13 +38,16 @@ inline void check(const int status) {
 
 /// Compiled form used by QCircuit::simulate/apply: environment QCLAB_FUSE_K =
 /// 0 (one plain sweep per gate), 1..5 (fused blocks of up to k qubits),
-/// "tiled" (tiled passes) or "auto" (the default: tiled from 22 qubits up,
-/// fused k = 5 below).
+/// "tiled" (interpreted tiled passes), "jit" (compiled tiled passes),
+/// "jitfixed" (compiled with the rotations' forms fixed: for circuits
+/// simulated repeatedly with the same angles) or "auto" (the default:
+/// compiled tiled passes from 22 qubits up, fused k = 5 below).
 inline int fuse_k() {
   const char* e = std::getenv("QCLAB_FUSE_K");
   if (!e || std::string(e) == "auto") return QSV_AUTO;
   if (std::string(e) == "tiled") return QSV_TILED;
   if (std::string(e) == "jit") return QSV_TILED_JIT;
+  if (std::string(e) == "jitfixed") return QSV_TILED_JIT_FIXED;
   const int k = std::atoi(e);
   return k < 0 ? 0 : (k > 5 ? 5 : k);
 }

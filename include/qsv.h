@@ -190,6 +190,14 @@ int qsv_dev_apply_gates(void* psi, int n, int dtype, const qsv_gate* gates,
  * qsv_cache_stats).  Creation fails with QSV_EUNSUP where hiprtc cannot be
  * loaded.                                                                 */
 #define QSV_TILED_JIT (-3)
+/* fuse_k == QSV_TILED_JIT_FIXED compiles the same passes as QSV_TILED_JIT
+ * with each rotation's scaled form (which of |cos| and |sin| is factored
+ * out of its 2x2) taken from its angle at creation, so the passes carry no
+ * branch per rotation: faster passes for circuits simulated repeatedly with
+ * the same angles.  Programs share compiled passes only where their
+ * rotations also share these forms (a new angle set usually compiles anew);
+ * parameter sweeps use QSV_TILED_JIT.  No counterpart in the reference.    */
+#define QSV_TILED_JIT_FIXED (-4)
 typedef struct qsv_program qsv_program;
 
 typedef struct qsv_program_info {
@@ -222,6 +230,11 @@ int qsv_program_run(const qsv_program* prog, void* psi, void* stream);
  * hosts without a device.                                                  */
 int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
                                 size_t* code_bytes, int64_t* npasses, uint64_t* source_hash);
+/* The same for fuse_k = QSV_TILED_JIT or QSV_TILED_JIT_FIXED (the check
+ * above is this one with QSV_TILED_JIT).  No counterpart in the reference. */
+int qsv_tiled_jit_compile_check_form(int n, int dtype, const qsv_gate* gates, size_t ngates,
+                                     int fuse_k, size_t* code_bytes, int64_t* npasses,
+                                     uint64_t* source_hash);
 int qsv_program_info_get(const qsv_program* prog, qsv_program_info* out);
 /* Launch i of one run (0-based, launch order): the qubits it acts on
  * (ascending, k of them), and its operator as interleaved complex doubles --

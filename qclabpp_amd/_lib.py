@@ -17,6 +17,7 @@ QSV_OK, QSV_EINVAL, QSV_EQUBIT, QSV_ENOMEM, QSV_EHIP, QSV_EDTYPE, QSV_ENODEV, QS
 QSV_TILED = -1  # fuse_k value selecting tiled passes (include/qsv.h)
 QSV_AUTO = -2   # fuse_k value letting the engine choose
 QSV_TILED_JIT = -3  # tiled passes compiled per circuit structure
+QSV_TILED_JIT_FIXED = -4  # the same with each rotation's scaled form fixed at creation
 QSV_C64, QSV_C128 = 1, 2
 
 
@@ -167,6 +168,8 @@ SIGNATURES = {
     "qsv_program_run": (_I, [_P, _P, _P]),
     "qsv_tiled_jit_compile_check": (_I, [_I, _I, _GP, _SZ, C.POINTER(_SZ), C.POINTER(C.c_int64),
                                          C.POINTER(C.c_uint64)]),
+    "qsv_tiled_jit_compile_check_form": (_I, [_I, _I, _GP, _SZ, _I, C.POINTER(_SZ), C.POINTER(C.c_int64),
+                                              C.POINTER(C.c_uint64)]),
     "qsv_program_info_get": (_I, [_P, C.POINTER(QsvProgramInfo)]),
     "qsv_program_op": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _DP]),
     "qsv_program_destroy": (_I, [_P]),

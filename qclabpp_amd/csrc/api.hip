@@ -29,6 +29,13 @@ int check_state_args(const void* psi, int n, int dtype) {
 
 int elem_bytes(int dtype) { return dtype == QSV_C128 ? 16 : 8; }
 
+int check_fuse_k(int fuse_k) {
+  if ((fuse_k >= 0 && fuse_k <= kMaxFuse) || fuse_k == QSV_TILED || fuse_k == QSV_AUTO ||
+      fuse_k == QSV_TILED_JIT || fuse_k == QSV_TILED_JIT_FIXED)
+    return QSV_OK;
+  return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT, QSV_TILED_JIT_FIXED or QSV_AUTO");
+}
+
 int apply_lowered(const KOp& op, void* psi, int n, int dtype, hipStream_t st) {
   PlainTuning tu;
   return dtype == QSV_C128 ? launch_plain<double>(op, psi, n, st, tu)
@@ -203,9 +210,7 @@ int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, i
   if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
   if (n < 1 || n > 40) return fail(QSV_EINVAL, "number of qubits must be in [1, 40]");
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
-  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
-      fuse_k != QSV_TILED_JIT)
-    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
+  QSV_TRY(check_fuse_k(fuse_k));
   if (diag_k < 0 || diag_k > kMaxDiag) return fail(QSV_EINVAL, "diag_k must be in [0, 10]");
   auto* prog = new (std::nothrow) qsv_program;
   if (!prog) return fail(QSV_ENOMEM, "host allocation failed");
@@ -220,6 +225,14 @@ int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, i
 
 int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t ngates,
                                 size_t* code_bytes, int64_t* npasses, uint64_t* source_hash) {
+  return qsv_tiled_jit_compile_check_form(n, dtype, gates, ngates, QSV_TILED_JIT, code_bytes, npasses,
+                                          source_hash);
+}
+
+int qsv_tiled_jit_compile_check_form(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,
+                                     size_t* code_bytes, int64_t* npasses, uint64_t* source_hash) {
+  if (fuse_k != QSV_TILED_JIT && fuse_k != QSV_TILED_JIT_FIXED)
+    return fail(QSV_EINVAL, "fuse_k must be QSV_TILED_JIT or QSV_TILED_JIT_FIXED");
   if (dtype != QSV_C64 && dtype != QSV_C128) return fail(QSV_EDTYPE, "dtype must be QSV_C64 or QSV_C128");
   if (n < kQTileBits || n > 40) return fail(QSV_EINVAL, "tiled passes need n in [12, 40]");
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
@@ -228,6 +241,7 @@ int qsv_tiled_jit_compile_check(int n, int dtype, const qsv_gate* gates, size_t 
   QProgram qp;
   QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qp, jit_tile_bits(dtype, n), true));
   qtile_scale_rotations(qp);  // as Program::build does
+  qp.fixed_forms = fuse_k == QSV_TILED_JIT_FIXED;
   size_t total = 0;
   uint64_t h = 1469598103934665603ull;  // FNV-1a over every pass's source
   for (size_t i = 0; i < qp.passes.size(); ++i) {
@@ -324,9 +338,7 @@ int qsv_dev_simulate(void* psi, int n, int dtype, const qsv_gate* gates, size_t 
                      int fuse_k, void* stream) {
   QSV_TRY(check_state_args(psi, n, dtype));
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
-  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
-      fuse_k != QSV_TILED_JIT)
-    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
+  QSV_TRY(check_fuse_k(fuse_k));
   Program prog;
   QSV_TRY(prog.build(n, dtype, gates, ngates, fuse_k, 0));
   auto st = static_cast<hipStream_t>(stream);
@@ -339,9 +351,7 @@ int qsv_host_simulate(void* host, int n, int dtype, const qsv_gate* gates, size_
                       int fuse_k) {
   QSV_TRY(check_state_args(host, n, dtype));
   if (ngates && !gates) return fail(QSV_EINVAL, "null gate array");
-  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
-      fuse_k != QSV_TILED_JIT)
-    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
+  QSV_TRY(check_fuse_k(fuse_k));
   Program prog;
   QSV_TRY(prog.build(n, dtype, gates, ngates, fuse_k, 0));
   const size_t bytes = (size_t(1) << n) * elem_bytes(dtype);
@@ -373,9 +383,7 @@ int qsv_host_circuit_matrix(void* mat, int k, int dtype, const qsv_gate* gates, 
                             int fuse_k) {
   if (!mat) return fail(QSV_EINVAL, "null matrix pointer");
   QSV_TRY(check_state_args(mat, 2 * k, dtype));
-  if ((fuse_k < 0 || fuse_k > kMaxFuse) && fuse_k != QSV_TILED && fuse_k != QSV_AUTO &&
-      fuse_k != QSV_TILED_JIT)
-    return fail(QSV_EINVAL, "fuse_k must be in [0, 5], QSV_TILED, QSV_TILED_JIT or QSV_AUTO");
+  QSV_TRY(check_fuse_k(fuse_k));
   std::vector<qsv_gate> recs;
   QSV_TRY(matrix_records(k, gates, ngates, recs));
   Program prog;

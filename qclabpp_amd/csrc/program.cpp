@@ -75,6 +75,8 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
   // hiprtc, without it the interpreting kernel runs the same passes
   if (fuse_k == QSV_AUTO)
     fuse_k = n >= kQTileBits + 10 ? (qtile_jit_available() ? QSV_TILED_JIT : QSV_TILED) : kMaxFuse;
+  const bool fixed_forms = fuse_k == QSV_TILED_JIT_FIXED;
+  if (fixed_forms) fuse_k = QSV_TILED_JIT;
   if ((fuse_k == QSV_TILED || fuse_k == QSV_TILED_JIT) && n < kQTileBits)
     fuse_k = kMaxFuse;  // one tile would be the whole state
   if (fuse_k == QSV_TILED || fuse_k == QSV_TILED_JIT) {
@@ -87,6 +89,7 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
     const int tb = jit ? jit_tile_bits(dtype, n) : kQTileBits;
     QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt, tb, jit));
     qtile_scale_rotations(qt);
+    qt.fixed_forms = fixed_forms;
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     qt_pass_off = 0;
     qt_phase_off = up(qt.passes.size() * sizeof(QPass));

@@ -104,6 +104,9 @@ struct QProgram {
   std::vector<QOp> ops;
   size_t gates_absorbed = 0;
   int tile_bits = kQTileBits;
+  // compiled passes only: each scaled rotation's form is fixed by its
+  // coefficients when the pass is generated (no branch on the form)
+  bool fixed_forms = false;
 };
 
 // Compile lowered, unfused gates (n >= tile_bits) into passes and phases.

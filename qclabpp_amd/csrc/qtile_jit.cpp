@@ -128,8 +128,11 @@ __device__ __forceinline__ void u_d2(C& a0, C& a1, C& a2, C& a3, const QC R* cf,
 
 // Op k applied to the register arrays `arrs`, one after the other (the
 // data-dependent scaled forms inside one branch).
+// fixed: the scaled forms' choice (which of cos and sin is larger) is taken
+// from the op's coefficients at generation time, so the op has no branch
+// (programs of the fixed-form kind only).
 void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128,
-             const std::vector<std::string>& arrs = {"v"}) {
+             const std::vector<std::string>& arrs = {"v"}, bool fixed = false) {
   // index of coefficient slot 0 of op k in the QOp array read as R[]
   const size_t stride = c128 ? sizeof(QOp) / sizeof(double) : sizeof(QOp) / sizeof(float);
   const size_t base = k * stride + (c128 ? offsetof(QOp, m) / sizeof(double) : offsetof(QOp, mf) / sizeof(float));
@@ -187,21 +190,32 @@ void emit_op(std::ostringstream& s, const QOp& o, size_t k, bool c128,
       // the form is data (it depends on the angle), so one compiled pass
       // serves every angle: a branch on a uniform value
       const bool rot = o.kind == Q_ROTS;
-      s << "      const R k = cf[" << at(0) << "];\n"
-        << "      if (cf[" << at(2) << "] != R(0)) {\n";
-      each([&] {
-        pairs([&](int p, int q, bool sw, bool) {
-          if (rot) s << "        u_ta(" << v(p) << ", " << v(q) << (sw ? ", -k);\n" : ", k);\n");
-          else s << "        u_xa(" << v(p) << ", " << v(q) << ", k);\n";
+      auto first_form = [&] {
+        each([&] {
+          pairs([&](int p, int q, bool sw, bool) {
+            if (rot) s << "        u_ta(" << v(p) << ", " << v(q) << (sw ? ", -k);\n" : ", k);\n");
+            else s << "        u_xa(" << v(p) << ", " << v(q) << ", k);\n";
+          });
         });
-      });
+      };
+      auto second_form = [&] {
+        each([&] {
+          pairs([&](int p, int q, bool sw, bool) {
+            if (rot) s << "        " << (sw ? "u_tbs(" : "u_tb(") << v(p) << ", " << v(q) << ", k);\n";
+            else s << "        u_xb(" << v(p) << ", " << v(q) << ", k);\n";
+          });
+        });
+      };
+      s << "      const R k = cf[" << at(0) << "];\n";
+      if (fixed) {
+        if ((c128 ? o.m[2] : double(o.mf[2])) != 0.0) first_form();
+        else second_form();
+        break;
+      }
+      s << "      if (cf[" << at(2) << "] != R(0)) {\n";
+      first_form();
       s << "      } else {\n";
-      each([&] {
-        pairs([&](int p, int q, bool sw, bool) {
-          if (rot) s << "        " << (sw ? "u_tbs(" : "u_tb(") << v(p) << ", " << v(q) << ", k);\n";
-          else s << "        u_xb(" << v(p) << ", " << v(q) << ", k);\n";
-        });
-      });
+      second_form();
       s << "      }\n";
       break;
     }
@@ -453,7 +467,7 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
       // the LDS tile is written next by the following tile's first phase
       if (!scatter) s << "      __syncthreads();\n";
     }
-    for (int o = Q.op_begin; o < Q.op_end; ++o) emit_op(s, qp.ops[o], size_t(o), c128);
+    for (int o = Q.op_begin; o < Q.op_end; ++o) emit_op(s, qp.ops[o], size_t(o), c128, {"v"}, qp.fixed_forms);
     if (scatter) {
       for (int i = 0; i < NR; ++i) s << "      tile[tp ^ " << Q.regpos_out[i] << "u] = v[" << i << "];\n";
       s << "      __syncthreads();\n";
@@ -760,7 +774,7 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
     s << "    {  // phase " << ph << "\n"
       << "      const QC R* cf = cf0;\n"
       << "      asm volatile(\"\" : \"+s\"(cf));\n";
-    for (int o = Q.op_begin; o < Q.op_end; ++o) emit_op(s, qp.ops[o], size_t(o), c128);
+    for (int o = Q.op_begin; o < Q.op_end; ++o) emit_op(s, qp.ops[o], size_t(o), c128, {"v"}, qp.fixed_forms);
     s << "    }\n";
     if (k + 1 < K) emit_exchange(s, sct[k], gat[k + 1], bin[k + 1], T);
   }

@@ -100,6 +100,7 @@ def simulate_host(vec: np.ndarray, gates) -> np.ndarray:
 TILED = _lib.QSV_TILED
 AUTO = _lib.QSV_AUTO
 TILED_JIT = _lib.QSV_TILED_JIT
+TILED_JIT_FIXED = _lib.QSV_TILED_JIT_FIXED
 
 
 def circuit_matrix(k: int, gates, dtype=None, fuse_k: int = AUTO, stream=None):

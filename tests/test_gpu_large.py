@@ -24,14 +24,15 @@ def _free():
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("fuse_k,depth", [(5, 20), (4, 20), (0, 3), (E.TILED, 20), (E.TILED_JIT, 20)])
+@pytest.mark.parametrize("fuse_k,depth", [(5, 20), (4, 20), (0, 3), (E.TILED, 20), (E.TILED_JIT, 20),
+                                           (E.TILED_JIT_FIXED, 20)])
 def test_brickwork30_c128_inverse_returns_to_zero(fuse_k, depth):
     import torch
     n = 30
     circ = W.brickwork(n, depth)
     fwd = E.Program(n, circ, fuse_k=fuse_k)
     back = E.Program(n, G.inverse(circ), fuse_k=fuse_k)
-    if fuse_k == E.TILED_JIT:  # compiled passes that relabel qubits, at full size
+    if fuse_k in (E.TILED_JIT, E.TILED_JIT_FIXED):  # compiled passes that relabel qubits, at full size
         assert fwd.info()["tiled_relabel"] > 0 and back.info()["tiled_relabel"] > 0
     psi = E.new_state(n, torch.complex128)
     fwd.run(psi)

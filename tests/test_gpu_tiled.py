@@ -16,7 +16,9 @@ from tests.parity import C64, C128, assert_close, tol
 
 pytestmark = pytest.mark.gpu
 
-FORMS = [pytest.param(E.TILED, id="interp"), pytest.param(E.TILED_JIT, id="jit")]
+FORMS = [pytest.param(E.TILED, id="interp"), pytest.param(E.TILED_JIT, id="jit"),
+         pytest.param(E.TILED_JIT_FIXED, id="jitfixed")]
+JIT_FORMS = [pytest.param(E.TILED_JIT, id="jit"), pytest.param(E.TILED_JIT_FIXED, id="jitfixed")]
 
 
 def run(prog, psi0):
@@ -68,7 +70,8 @@ def test_tiled_every_kind_c64(n, form):
 
 
 @pytest.mark.parametrize("form,count", [pytest.param(E.TILED, 5000, id="interp"),
-                                        pytest.param(E.TILED_JIT, 600, id="jit")])
+                                        pytest.param(E.TILED_JIT, 600, id="jit"),
+                                        pytest.param(E.TILED_JIT_FIXED, 600, id="jitfixed")])
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 def test_tiled_deep_pass_keeps_scale(dt, form, count):
     """Thousands of rotations on a 12-qubit state are one pass (its tile
@@ -168,28 +171,30 @@ def _structured(n):
             "line_then_high": c_line_then_hi}
 
 
+@pytest.mark.parametrize("form", JIT_FORMS)
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 @pytest.mark.parametrize("name", ["line_only", "one_gate", "high_then_line", "line_then_high"])
-def test_jit_load_store_routes(name, dt):
+def test_jit_load_store_routes(name, dt, form):
     n = 15
     circ = _structured(n)[name]
     psi = random_state(n, 12, np.complex128)
     want = oracle.apply(n, circ, psi)
     x = psi if dt == "c128" else psi.astype(np.complex64)
-    got = run(E.Program(n, circ, fuse_k=E.TILED_JIT, dtype=dt), x)
+    got = run(E.Program(n, circ, fuse_k=form, dtype=dt), x)
     assert_close(got, want, tol(dt), f"{name} {dt}")
 
 
+@pytest.mark.parametrize("form", JIT_FORMS)
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 @pytest.mark.parametrize("n", [18, 20, 22])
-def test_jit_relabeling_passes(n, dt):
+def test_jit_relabeling_passes(n, dt, form):
     """Compiled passes that move qubits onto the line bits (and back by the
     end of the program): brickwork plus every gate kind, against the
     oracle, with at least one relabeling pass in the schedule."""
     circ = W.brickwork(n, 20, seed=30 + n) + every_kind(n, n) + W.brickwork(n, 6, seed=n)
     psi = random_state(n, 40 + n, np.complex128)
     want = oracle.apply(n, circ, psi)
-    prog = E.Program(n, circ, fuse_k=E.TILED_JIT, dtype=dt)
+    prog = E.Program(n, circ, fuse_k=form, dtype=dt)
     info = prog.info()
     print(f"relabel n={n} {dt}: passes={info['nlaunch']} relabeling={info['tiled_relabel']}", flush=True)
     assert info["tiled_relabel"] > 0

@@ -46,6 +46,24 @@ def _check(n, circ, dtype):
     return _compile(n, circ, dtype)[0]
 
 
+def _form_check(n, circ, dtype, form, compile_=True):
+    """(pass count, source hash) of QSV_TILED_JIT or QSV_TILED_JIT_FIXED,
+    compiling every pass when compile_ is set."""
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("engine library not built")
+    arr, cnt = G.records(circ)
+    size = C.c_size_t()
+    npass = C.c_int64()
+    h = C.c_uint64()
+    st = _lib.lib().qsv_tiled_jit_compile_check_form(n, dtype, arr, cnt, form,
+                                                     C.byref(size) if compile_ else None,
+                                                     C.byref(npass), C.byref(h))
+    if st == _lib.QSV_EUNSUP and b"hiprtc" in _lib.lib().qsv_last_error():
+        pytest.skip("hiprtc not loadable here")
+    _lib.check(st)
+    return npass.value, h.value
+
+
 @pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
 def test_generic_angles_share_compiled_code(dtype):
     """Angles are data: two circuits of one structure whose generic angles
@@ -182,3 +200,42 @@ def test_adjoint_passes_compile(dtype):
     tf = W.tfim_ansatz(16, 3)
     npass, rows, size = _adjoint_compile(16, tf, dtype)
     assert rows == len(tf) and npass >= 2
+
+
+@pytest.mark.parametrize("dtype", [_lib.QSV_C128, _lib.QSV_C64])
+def test_fixed_form_passes_compile(dtype):
+    """QSV_TILED_JIT_FIXED emits each scaled rotation in the form its angle
+    picks, without the branch: every pass of a brickwork and of a circuit
+    with every gate kind compiles."""
+    from qclabpp_amd import engine as E
+    _form_check(14, W.brickwork(14, 8, seed=3), dtype, E.TILED_JIT_FIXED)
+    _form_check(13, every_kind(13, 13), dtype, E.TILED_JIT_FIXED)
+
+
+def test_fixed_form_source_follows_the_forms():
+    """Fixed-form passes depend on which of |cos| and |sin| is the larger
+    per rotation, not on the angles otherwise: angles that keep every form
+    share the source, a flipped form changes it, and the angle-generic
+    source is the same for all three angle sets."""
+    from qclabpp_amd import engine as E
+    n = 12
+
+    def circ(angles):
+        out = []
+        for layer in range(4):
+            for q in range(n):
+                out.append(G.RotationY(q, angles[layer * n + q]))
+            for i in range(layer % 2, n - 1, 2):
+                out.append(G.CX(i, i + 1))
+        return out
+
+    base = [0.3 + 0.01 * i for i in range(4 * n)]           # |cos| > |sin| everywhere
+    same = [0.2 + 0.005 * i for i in range(4 * n)]          # the same forms
+    flip = list(base)
+    flip[5] = 2.0                                            # |sin| > |cos| for one rotation
+    f = [_form_check(n, circ(a), _lib.QSV_C128, E.TILED_JIT_FIXED, compile_=False)[1] for a in (base, same, flip)]
+    assert f[0] == f[1] and f[0] != f[2]
+    g = {_form_check(n, circ(a), _lib.QSV_C128, E.TILED_JIT, compile_=False)[1] for a in (base, same, flip)}
+    assert len(g) == 1
+    assert _lib.lib().qsv_tiled_jit_compile_check_form(n, _lib.QSV_C128, None, 0, 0, None, None,
+                                                       None) == _lib.QSV_EINVAL
