@@ -71,6 +71,9 @@ def parse():
                     help="print a markdown table of the kernels in a rocprofv3 results database "
                          "(with --plain-json: the plain leg's launches joined to their algorithmic bytes)")
     ap.add_argument("--plain-json", default="", help="the JSON line of a `--legs plain` run")
+    ap.add_argument("--traffic-table", default="",
+                    help="FETCH_DB,WRITE_DB: rocprofv3 databases of two --kernels-only counter runs "
+                         "(FETCH_SIZE, WRITE_SIZE) -> HBM bytes per dispatch against the algorithmic bytes")
     ap.add_argument("--launches", default="",
                     help="with --trace-table: KERNEL:N:FETCH_DB:WRITE_DB -- the last N dispatches of "
                          "KERNEL, one row each, with the bytes the two counter runs' last N dispatches "
@@ -94,9 +97,10 @@ def dominant_programs(E, G, n):
 
 def kernels_only(n):
     """Launch pattern for counter collection: 2 x the compiled tiled brickwork
-    program (its pass kernels qtp_*), 2 x the interpreted one (k_qtile),
-    5 x k=5 block, 5 x dense 1-qubit sweep, 3 x copy of 2^n complex128
-    (2^n * 16 bytes read and written)."""
+    program (its pass kernels qtp_*), 2 x the same with fixed rotation forms
+    (qtf_*), 2 x the interpreted one (k_qtile), 5 x k=5 block, 5 x dense
+    1-qubit sweep, 3 x copy of 2^n complex128 (2^n * 16 bytes read and
+    written)."""
     import torch
 
     from qclabpp_amd import engine as E
@@ -104,10 +108,13 @@ def kernels_only(n):
     from qclabpp_amd import workloads as W
     blk, one = dominant_programs(E, G, n)
     jit = E.Program(n, W.brickwork(n, 20), fuse_k=E.TILED_JIT)
+    jitf = E.Program(n, W.brickwork(n, 20), fuse_k=E.TILED_JIT_FIXED)
     tiled = E.Program(n, W.brickwork(n, 20), fuse_k=E.TILED)
     psi = E.new_state(n, torch.complex128)
     for _ in range(2):
         jit.run(psi)
+    for _ in range(2):
+        jitf.run(psi)
     for _ in range(2):
         tiled.run(psi)
     for _ in range(5):
@@ -360,6 +367,49 @@ def launch_table(db_path, spec):
     print(f"\n{len(rows)} launches: {tot_ms:.1f} ms of kernels, {tot_b / 1e9:.2f} GB, "
           f"{tot_b / (tot_ms / 1e3) / 1e12:.2f} TB/s; span from the first start to the last end "
           f"{(rows[-1][2] - rows[0][1]) / 1e6:.1f} ms.")
+    return 0
+
+
+def traffic_table(fetch_db, write_db, n=30):
+    """HBM traffic per dispatch from a FETCH_SIZE and a WRITE_SIZE counter
+    database of `--kernels-only` (KiB per dispatch, averaged per kernel; the
+    compiled passes pooled as qtp_* and qtf_*).  FETCH_SIZE is scaled so that
+    the 16 GiB copy reads 16 GiB, WRITE_SIZE so that it writes 16 GiB; the
+    scaled bytes are set against the algorithmic 2 * 2^n * 16 bytes of every
+    kernel of that run."""
+    import sqlite3
+
+    def per_kernel(path):
+        acc = {}
+        for k, v in sqlite3.connect(path).execute(
+                "select kernel_name, value from counters_collection"):
+            nm = k.replace("HIP_vector_type<double, 2u>", "double2").replace(
+                "qsv::(anonymous namespace)::", "").split("(")[0]
+            nm = "qtp_*" if nm.startswith("qtp_") else "qtf_*" if nm.startswith("qtf_") else nm
+            a = acc.setdefault(nm, [0.0, 0])
+            a[0] += v
+            a[1] += 1
+        return {k: (t / c, c) for k, (t, c) in acc.items()}
+
+    fetch, write = per_kernel(fetch_db), per_kernel(write_db)
+    copy = next(k for k in fetch if "copyBuffer" in k)
+    alg = 2.0 * (1 << n) * 16
+    fs, ws = alg / 2 / (fetch[copy][0] * 1024), alg / 2 / (write[copy][0] * 1024)
+    print(f"Scale factors from `{copy}` (2^{n} * 16 B read and written): FETCH x {fs:.4f}, WRITE x {ws:.4f}\n")
+    print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled bytes | algorithmic bytes | ratio |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    lines = []
+    for k in sorted(fetch, key=lambda x: (x == copy, x)):
+        if k not in write:
+            continue
+        f, c = fetch[k]
+        w, _ = write[k]
+        b = f * 1024 * fs + w * 1024 * ws
+        print(f"| `{k}` | {c} | {f:.0f} | {w:.0f} | {b:.4e} | {alg:.4e} | {b / alg:.4f} |")
+        if k != copy:
+            lines.append(f"traffic_bytes_per_launch {k} n={n}: {b:.6e}")
+    print()
+    print("\n".join(lines))
     return 0
 
 
@@ -664,6 +714,9 @@ def main():
         return 2
     if args.kernels_only:
         return kernels_only(args.n)
+    if args.traffic_table:
+        fdb, wdb = args.traffic_table.split(",")
+        return traffic_table(fdb, wdb, args.n)
     if args.trace_table and args.launches:
         return launch_table(args.trace_table, args.launches)
     if args.trace_table:
@@ -834,9 +887,11 @@ def main():
         # 78.6 TFLOP/s counts an FMA as two, so the issue peak is 39.3e12
         lane_ops = ji["fp_lane_ops"] / npass
         out["roofline"] = {"bound": "hbm", "achieved": gbps_pass, "peak": PEAK_HBM, "unit": "GB/s",
-                           "frac": gbps_pass / PEAK_HBM, "traffic": measured_traffic(n, "qtp_*"),
+                           "frac": gbps_pass / PEAK_HBM,
+                           "traffic": measured_traffic(n, "qtf_*" if best == "tiled_jit_fixed" else "qtp_*"),
                            "traffic_source": TRAFFIC_SOURCE,
-                           "kernel": f"qtp_* (compiled tiled passes, {best} form, mean per launch)",
+                           "kernel": ("qtf_*" if best == "tiled_jit_fixed" else "qtp_*")
+                           + f" (compiled tiled passes, {best} form, mean per launch)",
                            "alg_bytes_per_launch": 2 * S,
                            "fp64_valu": {"lane_ops_per_launch": lane_ops,
                                          "achieved": lane_ops / (ms_pass / 1e3) / 1e12,

@@ -82,11 +82,10 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
   if (fuse_k == QSV_TILED || fuse_k == QSV_TILED_JIT) {
     tiled = true;
     jit = fuse_k == QSV_TILED_JIT;
-    // compiled complex64 passes use 2^13-amplitude tiles (64 KiB of LDS, as
-    // a complex128 tile of 2^12): 18 passes instead of 21 on the n=30
-    // brickwork, 10% faster; complex128 stays at 2^12 (a 2^13 tile would
-    // leave one block per CU, measured 3% slower despite 15 passes for 17)
-    const int tb = jit ? jit_tile_bits(dtype, n) : kQTileBits;
+    // tile size of compiled passes: jit_tile_bits (2^13 for complex64 and
+    // fixed-form complex128 passes, 2^12 for complex128 passes that branch
+    // per rotation)
+    const int tb = jit ? jit_tile_bits(dtype, n, fixed_forms) : kQTileBits;
     QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt, tb, jit));
     qtile_scale_rotations(qt);
     qt.fixed_forms = fixed_forms;

@@ -331,6 +331,10 @@ uint32_t line_mask(const uint8_t* tb, int T, bool c128) {
   return m;
 }
 
+// Kernel names: qtp_<pass>, or qtf_<pass> for fixed-form passes (so that
+// profiles keep the two forms apart).
+const char* pass_prefix(const QProgram& qp) { return qp.fixed_forms ? "qtf_" : "qtp_"; }
+
 // A pass as one kernel.  Each tile goes HBM -> phases -> HBM.  A phase
 // gathers its 16 register amplitudes from the LDS tile, applies its ops and
 // writes them back; the first phase instead loads its registers straight
@@ -378,7 +382,7 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
   // (64 KiB, 2 blocks of 8 waves)
   const int wpe = c128 ? (T == 12 ? 2 : 1) : (T == 12 ? 3 : 4);
   s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(" << wpe << ")))\n"
-    << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
+    << "void " << pass_prefix(qp) << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
   if (!(din && dout && K == 1)) s << "  __shared__ C tile[" << (1u << T) << "];\n";
   s << "  const unsigned tid = threadIdx.x;\n"
     << "  auto tile_base = [&](unsigned long long t) {\n"
@@ -505,10 +509,10 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
 // base slot plus a literal (one address register per side, immediate
 // offsets).  Bank rules measured on gfx950 with a probe kernel: ds_read_b128
 // serves 32 lanes at a time and is conflict-free when their 16-byte slots
-// differ mod 32; ds_write_b128 serves 16 lanes, slots distinct mod 16.  With
+// differ mod 32; ds_write_b128 serves 8 lanes, slots distinct mod 8.  With
 // this padding a compressed bit m moves the slot by 2^(m mod 5) mod 32 (its
 // "class"), so reads are conflict-free when lane bits 0-4 hold the five
-// classes, and writes when lane bits 0-3 hold classes 0-3.
+// classes, and writes when lane bits 0-2 hold classes 0-2.
 
 // One side of a phase change: thread bit j -> local bit ord[j]; register i
 // at local position offset off[i] (register bits only).
@@ -534,7 +538,7 @@ int xclass(int m, int b, int T) { return xbit(m, b, T) % 5; }
 
 // Conflict-free lanes of an order under split bit b: reads count the lane
 // bits 0-4 of distinct classes (5 = conflict-free), writes whether lane
-// bits 0-3 hold classes 0-3.
+// bits 0-2 hold classes 0-2.
 int read_lanes_ok(const std::vector<int>& ord, int b, int T) {
   unsigned seen = 0;
   int n = 0;
@@ -547,10 +551,10 @@ int read_lanes_ok(const std::vector<int>& ord, int b, int T) {
 }
 bool write_lanes_ok(const std::vector<int>& ord, int b, int T) {
   unsigned seen = 0;
-  for (size_t j = 0; j < 4 && j < ord.size(); ++j) {
+  for (size_t j = 0; j < 3 && j < ord.size(); ++j) {
     if (ord[j] == b) continue;  // that lane bit sits out each round in turn
     const int c = xclass(ord[j], b, T);
-    if (c == 4 || (seen >> c & 1u)) return false;
+    if (c > 2 || (seen >> c & 1u)) return false;
     seen |= 1u << c;
   }
   return true;
@@ -559,9 +563,10 @@ bool write_lanes_ok(const std::vector<int>& ord, int b, int T) {
 // Thread order of a phase of the split kernels: as lane_order, with the
 // classes of the padded compressed slots (split bit b of the phase change
 // that fills the registers; b < 0 for a phase loaded from HBM): lane bits
-// 0-3 take classes 0-3, lane bit 4 class 4, after the lines a direct first
-// phase loads; the lines a direct last phase stores stay among the 6 lane
-// bits.
+// 0-2 take classes 0-2 and lane bits 3-4 classes 3-4 (after the lines a
+// direct first phase loads, local bits 0-2: classes 0-2), preferring bits
+// other than the top tile bit, whose class moves with the split bit; the
+// lines a direct last phase stores stay among the 6 lane bits.
 std::vector<int> lane_order_split(int T, uint32_t regmask, uint32_t first, uint32_t lanes, int b) {
   std::vector<int> ord;
   auto free_bit = [&](int m) {
@@ -577,14 +582,16 @@ std::vector<int> lane_order_split(int T, uint32_t regmask, uint32_t first, uint3
   while (ord.size() < 5) {
     auto wanted = [&](int m) {
       const int c = xclass(m, b, T);
-      return !used(c) && (ord.size() < 4 ? c < 4 : c == 4);
+      return !used(c) && (ord.size() < 3 ? c < 3 : c >= 3);
     };
     int pick = -1;
-    for (int pass = 0; pass < 4 && pick < 0; ++pass)
+    for (int pass = 0; pass < 6 && pick < 0; ++pass)
       for (int m = 0; m < T && pick < 0; ++m) {
         if (!free_bit(m)) continue;
-        if ((pass == 0 && wanted(m) && (lanes >> m & 1u)) || (pass == 1 && wanted(m)) ||
-            (pass == 2 && !used(xclass(m, b, T))) || pass == 3)
+        const bool top = m == T - 1;
+        if ((pass == 0 && wanted(m) && !top && (lanes >> m & 1u)) || (pass == 1 && wanted(m) && !top) ||
+            (pass == 2 && wanted(m)) || (pass == 3 && !used(xclass(m, b, T)) && !top) ||
+            (pass == 4 && !used(xclass(m, b, T))) || pass == 5)
           pick = m;
       }
     if (pick < 0) break;
@@ -725,8 +732,14 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
     out_ofs[i] = gofs(P.tbits_out, T, last.off[i]);
   }
   const bool lds = !din || !dout || K > 1;
-  s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(3)))\n"
-    << "void qtp_" << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
+  // waves per SIMD the register budget is sized for: four blocks of 2^12 or
+  // two of 2^13 amplitudes fit a CU's LDS (16 waves); fixed-form passes fit
+  // the 128 registers of four waves per SIMD, passes that branch per
+  // rotation need about 166 and run three
+  const int wpe = qp.fixed_forms ? 4 : 3;
+  s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(" << wpe
+    << ")))\n"
+    << "void " << pass_prefix(qp) << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
   if (lds) s << "  __shared__ C tile[" << xpad((1u << (T - 1)) - 1) + 1 << "];\n";
   s << "  const unsigned tid = threadIdx.x;\n"
     << "  auto tile_base = [&](unsigned long long t) {\n"
@@ -791,10 +804,10 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
   return true;
 }
 
-// Complex128 passes of 2^12-amplitude tiles use the half buffer unless
-// QSV_JIT_SPLIT=0 (full-tile kernels, two waves per SIMD: for comparison).
+// Complex128 passes use the half buffer unless QSV_JIT_SPLIT=0 (full-tile
+// kernels: for comparison).
 bool use_split(const QProgram& qp, bool c128) {
-  if (!c128 || qp.tile_bits != kQTileBits) return false;
+  if (!c128) return false;
   const char* e = std::getenv("QSV_JIT_SPLIT");
   return !(e && e[0] == '0');
 }
@@ -1442,7 +1455,7 @@ int build_modules(const QProgram& qp, const char* prefix, Gen gen, QJit& out) {
 }  // namespace
 
 int qtile_jit_build(const QProgram& qp, bool c128, QJit& out) {
-  return build_modules(qp, "qtp_", [&](size_t p0, size_t p1) { return qtile_jit_source(qp, c128, p0, p1); },
+  return build_modules(qp, pass_prefix(qp), [&](size_t p0, size_t p1) { return qtile_jit_source(qp, c128, p0, p1); },
                        out);
 }
 
