@@ -29,7 +29,10 @@ int device_cus();  // qtile.hip
 namespace {
 
 constexpr int NR = 1 << kQRegBits;
-constexpr int kJitBlocksPerCU = 32;  // as the interpreted pass kernel
+// blocks per CU of a pass launch (each walks a contiguous range of tiles):
+// on the n=30 brickwork 32 gave 111 ms (branching passes) and 100 ms (fixed
+// forms), 128 gave 106 and 99 ms; 2-8 (near-persistent grids) 104-128 ms
+constexpr int kJitBlocksPerCU = 128;
 
 static_assert(sizeof(QOp) % sizeof(double) == 0 && offsetof(QOp, m) % sizeof(double) == 0 &&
                   offsetof(QOp, mf) % sizeof(float) == 0,
