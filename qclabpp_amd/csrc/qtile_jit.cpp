@@ -29,10 +29,14 @@ int device_cus();  // qtile.hip
 namespace {
 
 constexpr int NR = 1 << kQRegBits;
-// blocks per CU of a pass launch (each walks a contiguous range of tiles):
-// on the n=30 brickwork 32 gave 111 ms (branching passes) and 100 ms (fixed
-// forms), 128 gave 106 and 99 ms; 2-8 (near-persistent grids) 104-128 ms
+// grid of a pass launch: each block walks a contiguous range of at least
+// kJitTilesPerBlock tiles, with at most kJitBlocksPerCU blocks per CU.  On
+// the n=30 brickwork (2^18 tiles of 2^12, 2^17 of 2^13) 32 blocks per CU gave
+// 111 ms (branching passes) and 100 ms (fixed forms), 128 gave 106 and 99
+// ms, 2-8 (near-persistent grids) 104-128 ms; at n=28, 128 blocks per CU
+// (two tiles each) took 32 ms against 26 ms for 32 (eight tiles each).
 constexpr int kJitBlocksPerCU = 128;
+constexpr uint64_t kJitTilesPerBlock = 8;
 
 static_assert(sizeof(QOp) % sizeof(double) == 0 && offsetof(QOp, m) % sizeof(double) == 0 &&
                   offsetof(QOp, mf) % sizeof(float) == 0,
@@ -552,12 +556,12 @@ int read_lanes_ok(const std::vector<int>& ord, int b, int T) {
   }
   return n;
 }
-bool write_lanes_ok(const std::vector<int>& ord, int b, int T) {
+bool write_lanes_ok(const std::vector<int>& ord, int b, int T, int wl) {
   unsigned seen = 0;
-  for (size_t j = 0; j < 3 && j < ord.size(); ++j) {
+  for (size_t j = 0; j < size_t(wl) && j < ord.size(); ++j) {
     if (ord[j] == b) continue;  // that lane bit sits out each round in turn
     const int c = xclass(ord[j], b, T);
-    if (c > 2 || (seen >> c & 1u)) return false;
+    if (c >= wl || (seen >> c & 1u)) return false;
     seen |= 1u << c;
   }
   return true;
@@ -570,7 +574,7 @@ bool write_lanes_ok(const std::vector<int>& ord, int b, int T) {
 // direct first phase loads, local bits 0-2: classes 0-2), preferring bits
 // other than the top tile bit, whose class moves with the split bit; the
 // lines a direct last phase stores stay among the 6 lane bits.
-std::vector<int> lane_order_split(int T, uint32_t regmask, uint32_t first, uint32_t lanes, int b) {
+std::vector<int> lane_order_split(int T, uint32_t regmask, uint32_t first, uint32_t lanes, int b, int wl) {
   std::vector<int> ord;
   auto free_bit = [&](int m) {
     return !(regmask >> m & 1u) && std::find(ord.begin(), ord.end(), m) == ord.end();
@@ -585,7 +589,7 @@ std::vector<int> lane_order_split(int T, uint32_t regmask, uint32_t first, uint3
   while (ord.size() < 5) {
     auto wanted = [&](int m) {
       const int c = xclass(m, b, T);
-      return !used(c) && (ord.size() < 3 ? c < 3 : c >= 3);
+      return !used(c) && (int(ord.size()) < wl ? c < wl : c >= wl);
     };
     int pick = -1;
     for (int pass = 0; pass < 6 && pick < 0; ++pass)
@@ -663,10 +667,13 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
   // a relabeling one-phase pass with both ends direct would have no barrier
   // between the loads and the stores of its block; it stores through LDS
   bool dout = !(L.regmask & line_out) && !(K == 1 && din && relabels);
+  // lane bits whose classes the writes constrain: 3 for ds_write_b128 (8
+  // lanes per cycle), 4 for ds_write_b64 (16 lanes of 8 bytes)
+  const int wl = c128 ? 3 : 4;
   // a direct store needs every written line bit among the 6 lane bits
   // (always the case for the schedules built here; else store through LDS)
   XSide tile_side;  // positions tid + threads*j
-  tile_side.ord = lane_order_split(T, tile_order, 0u, 0u, -1);
+  tile_side.ord = lane_order_split(T, tile_order, 0u, 0u, -1, wl);
   tile_side.regmask = tile_order;
   for (int i = 0; i < NR; ++i) tile_side.off[i] = threads * uint32_t(i);
   // each phase at its gather and at its write-back
@@ -693,9 +700,9 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
     long best_score = -1;
     for (int b = T - 1; b >= 0; --b) {
       if (!(dmask >> b & 1u)) continue;
-      std::vector<int> ord = dmask == tile_order ? tile_side.ord : lane_order_split(T, dmask, 0u, lanes, b);
+      std::vector<int> ord = dmask == tile_order ? tile_side.ord : lane_order_split(T, dmask, 0u, lanes, b, wl);
       const long score = ((src.regmask >> b & 1u) ? 100 : 0) + 10 * read_lanes_ok(ord, b, T) +
-                         (write_lanes_ok(src.ord, b, T) ? 1 : 0);
+                         (write_lanes_ok(src.ord, b, T, wl) ? 1 : 0);
       if (score > best_score) {
         best_score = score;
         best = b;
@@ -709,7 +716,7 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
       const QPhase& Q = qp.phases[P.phase_begin + k];
       const uint32_t lanes = k == K - 1 && dout ? line_out : 0u;
       if (k == 0 && din) {
-        gat[0].ord = lane_order_split(T, Q.regmask, line_in, lanes, -1);
+        gat[0].ord = lane_order_split(T, Q.regmask, line_in, lanes, -1, wl);
       } else {
         const XSide& src = k == 0 ? tile_side : sct[k - 1];
         bin[k] = choose(src, Q.regmask, lanes, gat[k].ord);
@@ -739,7 +746,7 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
   // two of 2^13 amplitudes fit a CU's LDS (16 waves); fixed-form passes fit
   // the 128 registers of four waves per SIMD, passes that branch per
   // rotation need about 166 and run three
-  const int wpe = qp.fixed_forms ? 4 : 3;
+  const int wpe = c128 ? (qp.fixed_forms ? 4 : 3) : (qp.fixed_forms ? 6 : 4);
   s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(" << wpe
     << ")))\n"
     << "void " << pass_prefix(qp) << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
@@ -807,10 +814,10 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
   return true;
 }
 
-// Complex128 passes use the half buffer unless QSV_JIT_SPLIT=0 (full-tile
-// kernels: for comparison).
-bool use_split(const QProgram& qp, bool c128) {
-  if (!c128) return false;
+// Compiled passes use the half buffer unless QSV_JIT_SPLIT=0 (full-tile
+// kernels: for comparison).  Complex64 (tiles of 2^13, 33 KiB per block):
+// the n=28 brickwork 20.6 -> 16.2 ms, n=33 666 -> 580 ms.
+bool use_split() {
   const char* e = std::getenv("QSV_JIT_SPLIT");
   return !(e && e[0] == '0');
 }
@@ -1309,7 +1316,7 @@ void qtile_jit_cache_stats(int64_t* mem_hits, int64_t* disk_hits, int64_t* compi
 std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1) {
   std::ostringstream s;
   s << "#define REAL " << (c128 ? "double" : "float") << "\n" << kPrelude;
-  const bool split = use_split(qp, c128);
+  const bool split = use_split();
   for (size_t i = p0; i < p1 && i < qp.passes.size(); ++i) {
     std::ostringstream k;
     if (split && emit_pass_split(k, qp, i, c128)) {
@@ -1499,7 +1506,8 @@ int qtile_jit_launch_adjoint(const QJit& j, size_t i, void* phi, void* lam, cons
 int qtile_jit_launch(const QJit& j, size_t i, void* psi, const QOp* ops, hipStream_t st) {
   if (i >= j.fn.size()) return fail(QSV_EINVAL, "tiled pass index out of range");
   uint64_t ntiles = j.ntiles[i];
-  const uint64_t want = uint64_t(device_cus()) * kJitBlocksPerCU;
+  const uint64_t want = std::min(uint64_t(device_cus()) * kJitBlocksPerCU,
+                                 std::max<uint64_t>(1, ntiles / kJitTilesPerBlock));
   const unsigned grid = unsigned(ntiles < want ? ntiles : want);
   const void* cf = ops;
   void* args[] = {&psi, &cf, &ntiles};

@@ -28,13 +28,13 @@ struct QJit {
   unsigned threads = kQThreads;   // block size: 2^(tile bits - register bits)
 };
 
-// Tile bits of compiled passes: 2^13-amplitude tiles for complex64 (64 KiB
-// of LDS, as a complex128 tile of 2^12) and for fixed-form complex128 passes
-// (a half-tile LDS buffer of 66 KiB: two blocks of eight waves per CU; the
-// n=30 brickwork takes 12 passes instead of 13, 98 ms against 103).  Passes
-// that branch per rotation need about 166 registers, so a 2^13 tile would
-// leave one block per CU: they keep 2^12.  QSV_JIT_TILE_BITS=12 or 13
-// overrides the choice.
+// Tile bits of compiled passes (their phase changes go through a half-tile
+// LDS buffer): 2^13-amplitude tiles for complex64 (33 KiB per block) and for
+// fixed-form complex128 passes (66 KiB: two blocks of eight waves per CU;
+// the n=30 brickwork takes 12 passes instead of 13, 98 ms against 103).
+// Complex128 passes that branch per rotation need about 166 registers, so a
+// 2^13 tile would leave one block per CU: they keep 2^12.
+// QSV_JIT_TILE_BITS=12 or 13 overrides the choice.
 inline int jit_tile_bits(int dtype, int n, bool fixed_forms = false) {
   int t = (dtype == QSV_C64 || fixed_forms) && n >= kQTileBitsMax ? kQTileBitsMax : kQTileBits;
   if (const char* e = std::getenv("QSV_JIT_TILE_BITS")) {
