@@ -480,8 +480,9 @@ def host_entry_leg(args, circ, ms_jit):
 def batch_leg(args):
     """512 hardware-efficient-ansatz circuits (n ~ U{8..22}, L ~ U{4..16},
     per layer RY+RZ on every qubit then a CZ ladder), complex64, from
-    |0...0> into one packed buffer: autotuned tile and fusion width, then
-    timed graph replays.  Reports circuits/s and the p50/p99 of per-circuit
+    |0...0> into one packed buffer: autotuned tile and fusion width, and
+    whether the larger circuits run register-phase passes, then timed graph
+    replays.  Reports circuits/s and the p50/p99 of per-circuit
     completion times within a run."""
     import numpy as np
     import torch
@@ -508,7 +509,8 @@ def batch_leg(args):
            "circuits_per_s": args.batch / (ms / 1e3), "gates_per_s": gates / (ms / 1e3),
            "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
            "tuned": best, "tune_s": tune_s,
-           "candidates": [{k: r[k] for k in ("tile_bits", "fuse_k", "ms")} for r in results],
+           "candidates": [{k: r[k] for k in ("tile_bits", "small_bits", "fuse_k", "phases", "ms")}
+                          for r in results],
            "max_abs_diff_between_candidates": max(r["max_abs_diff"] for r in results),
            "passes": b.info["npasses"], "levels": b.info["nlevels"], "small_circuits": b.info["nsmall"],
            "roofline": {"bound": "hbm", "achieved": b.info["alg_bytes"] / (ms / 1e3) / 1e9, "peak": PEAK_HBM,

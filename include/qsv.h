@@ -324,7 +324,12 @@ typedef struct qsv_batch_opts {
   int32_t small_bits;  /* circuits with n <= small_bits run whole in LDS (0 = largest) */
   int32_t fuse_k;      /* 0: gates as given; 1: merge runs of 1-qubit gates;
                           2: also fuse into blocks of at most 2 qubits           */
-  int32_t reserved;
+  int32_t phases;      /* 0: circuits above small_bits run tile passes that apply
+                          one gate per LDS round trip; 1: they run register-
+                          phase passes of 2^12-amplitude tiles (16 amplitudes
+                          per thread, gates on the register bits, the unfused
+                          gates of the circuit); circuits up to
+                          max(small_bits, 11) qubits stay whole in LDS        */
 } qsv_batch_opts;
 
 typedef struct qsv_batch_info_t {
@@ -357,7 +362,8 @@ int qsv_batch_run(qsv_batch* batch, void* packed, int use_graph, void* stream);
 /* Per-circuit completion time of the last run, seconds from its start.    */
 int qsv_batch_latencies(qsv_batch* batch, double* seconds, void* stream);
 int qsv_batch_destroy(qsv_batch* batch);
-/* Time every candidate (tile_bits in 10..13 that fit, fuse_k in 0..2) on
+/* Time every candidate (tile_bits in 10..13 that fit, fuse_k in 0..2, and
+ * register-phase passes with small_bits at its largest and at 11) on
  * `packed` (reps runs each, graph replay), check that all candidates give
  * the same states (max |difference| reported per candidate), and return the
  * fastest in *best.  `scratch` holds a copy of the first candidate's states

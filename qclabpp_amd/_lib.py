@@ -54,7 +54,7 @@ class QsvBatchOpts(C.Structure):
     """Mirror of `qsv_batch_opts` (include/qsv.h)."""
 
     _fields_ = [("tile_bits", C.c_int32), ("small_bits", C.c_int32), ("fuse_k", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("phases", C.c_int32)]
 
 
 class QsvBatchInfo(C.Structure):

@@ -29,20 +29,20 @@ def _pack(circuits):
     return arr, offs
 
 
-def _opts(tile_bits=0, small_bits=0, fuse_k=1):
+def _opts(tile_bits=0, small_bits=0, fuse_k=1, phases=0):
     o = QsvBatchOpts()
-    o.tile_bits, o.small_bits, o.fuse_k = tile_bits, small_bits, fuse_k
+    o.tile_bits, o.small_bits, o.fuse_k, o.phases = tile_bits, small_bits, fuse_k, phases
     return o
 
 
 class Batch:
     def __init__(self, circuits, nqubits, dtype: str = "c64", tile_bits: int = 0,
-                 small_bits: int = 0, fuse_k: int = 1):
+                 small_bits: int = 0, fuse_k: int = 1, phases: int = 0):
         self.dtype = QSV_C64 if dtype in ("c64", "complex64") else QSV_C128
         self.nq = np.ascontiguousarray(np.asarray(nqubits, dtype=np.int32))
         self._arr, self._goff = _pack(circuits)
         goff = self._goff.astype(np.uintp)
-        o = _opts(tile_bits, small_bits, fuse_k)
+        o = _opts(tile_bits, small_bits, fuse_k, phases)
         h = C.c_void_p()
         check(lib().qsv_batch_create(self.dtype, len(circuits),
                                      self.nq.ctypes.data_as(C.POINTER(C.c_int)), self._arr,
@@ -57,7 +57,8 @@ class Batch:
                      "nops": inf.nops, "nlevels": inf.nlevels,
                      "total_amplitudes": inf.total_amplitudes,
                      "tile_bits": inf.opts.tile_bits, "small_bits": inf.opts.small_bits,
-                     "fuse_k": inf.opts.fuse_k, "alg_bytes": inf.alg_bytes}
+                     "fuse_k": inf.opts.fuse_k, "phases": inf.opts.phases,
+                     "alg_bytes": inf.alg_bytes}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -87,7 +88,8 @@ class Batch:
 
 
 def autotune(circuits, nqubits, packed, scratch, dtype: str = "c64", reps: int = 3, stream=None):
-    """Time every (tile_bits, fuse_k) candidate; returns (best opts dict, results)."""
+    """Time every (tile_bits, small_bits, fuse_k, phases) candidate; returns
+    (best opts dict, results)."""
     dt = QSV_C64 if dtype in ("c64", "complex64") else QSV_C128
     nq = np.ascontiguousarray(np.asarray(nqubits, dtype=np.int32))
     arr, goff = _pack(circuits)
@@ -99,9 +101,11 @@ def autotune(circuits, nqubits, packed, scratch, dtype: str = "c64", reps: int =
                                    goff.ctypes.data_as(C.POINTER(C.c_size_t)),
                                    C.c_void_p(packed.data_ptr()), C.c_void_p(scratch.data_ptr()),
                                    reps, C.byref(best), res, 32, C.byref(nres), stream_handle(stream)))
-    results = [{"tile_bits": r.opts.tile_bits, "fuse_k": r.opts.fuse_k, "ms": r.ms,
+    results = [{"tile_bits": r.opts.tile_bits, "small_bits": r.opts.small_bits,
+                "fuse_k": r.opts.fuse_k, "phases": r.opts.phases, "ms": r.ms,
                 "max_abs_diff": r.max_abs_diff} for r in res[:nres.value]]
-    return {"tile_bits": best.tile_bits, "small_bits": best.small_bits, "fuse_k": best.fuse_k}, results
+    return {"tile_bits": best.tile_bits, "small_bits": best.small_bits, "fuse_k": best.fuse_k,
+            "phases": best.phases}, results
 
 
 def signature(circuits, nqubits, dtype: str) -> str:

@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "fusion.hpp"
 #include "tile.hpp"
+#include "qtile.hpp"
 #include "tile_launch.hpp"
 
 namespace qsv {
@@ -64,9 +65,15 @@ struct Batch {
   std::vector<TPass> passes;
   std::vector<TOp> ops;
   std::vector<TItem> items;
+  // register-phase passes of the circuits above small_bits (opts.phases)
+  std::vector<QPass> qpasses;
+  std::vector<QPhase> qphases;
+  std::vector<QOp> qops;
+  std::vector<QPassInfo> qpinfo;
   Level small;
   std::vector<Level> large;
   DevBuf d_circs, d_passes, d_ops, d_items, d_tickets, d_done, d_stamps;
+  DevBuf d_qpasses, d_qphases, d_qops, d_qpinfo;
   // graph
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -114,6 +121,9 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
   if (opts.tile_bits < 6 || opts.tile_bits > cap) return fail(QSV_EINVAL, "tile_bits out of range");
   if (opts.small_bits < 1 || opts.small_bits > cap) return fail(QSV_EINVAL, "small_bits out of range");
   if (opts.fuse_k < 0 || opts.fuse_k > 2) return fail(QSV_EINVAL, "batch fuse_k must be 0, 1 or 2");
+  if (opts.phases != 0 && opts.phases != 1) return fail(QSV_EINVAL, "batch phases must be 0 or 1");
+  // register-phase passes take tiles of 2^kQTileBits: smaller circuits stay whole in LDS
+  const int small_cut = opts.phases ? std::max(opts.small_bits, kQTileBits - 1) : opts.small_bits;
   nq.assign(nqubits, nqubits + ncirc);
   offsets.assign(ncirc + 1, 0);
   for (int c = 0; c < ncirc; ++c) {
@@ -130,6 +140,34 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
       QSV_TRY(lower_gate(gates[g], n, k));
       lowered.push_back(k);
     }
+    const bool is_small = n <= small_cut;
+    if (!is_small && opts.phases) {
+      // passes and register phases of the interpreted tiled kernel, from the
+      // unfused gates; descriptors appended to the batch's arrays
+      QProgram qp;
+      QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qp, kQTileBits, false));
+      qtile_scale_rotations(qp);
+      const int pb = int(qpasses.size()), phb = int(qphases.size()), ob = int(qops.size());
+      for (QPhase ph : qp.phases) {
+        ph.op_begin += ob;
+        ph.op_end += ob;
+        qphases.push_back(ph);
+      }
+      const int np = int(qp.passes.size());
+      for (int k = 0; k < np; ++k) {
+        QPass P = qp.passes[k];
+        P.phase_begin += phb;
+        P.phase_end += phb;
+        qpasses.push_back(P);
+        qpinfo.push_back(QPassInfo{c, k == 0, k == np - 1, 0});
+      }
+      qops.insert(qops.end(), qp.ops.begin(), qp.ops.end());
+      first_pass[c] = pb;
+      npass[c] = np;
+      circs.push_back(TCircuit{offsets[c], n, np});
+      large_ids.push_back(c);
+      continue;
+    }
     std::vector<KOp> use;
     if (opts.fuse_k == 2) {
       FusionOptions fo;
@@ -140,7 +178,6 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
     } else {
       use = lowered;
     }
-    const bool is_small = n <= opts.small_bits;
     first_pass[c] = int(passes.size());
     QSV_TRY(compile_tiled(use, n, is_small ? n : opts.tile_bits, opts.fuse_k >= 1, c, passes, ops));
     npass[c] = int(passes.size()) - first_pass[c];
@@ -167,9 +204,10 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
     L.item_begin = items.size();
     for (int c : large_ids) {
       if (k >= npass[c]) continue;
-      const TPass& P = passes[first_pass[c] + k];
-      for (uint64_t t = 0; t < P.ntiles; ++t) items.push_back(TItem{first_pass[c] + k, 0, t});
-      L.tmax = std::max(L.tmax, P.t);
+      const int pi = first_pass[c] + k;
+      const uint64_t nt = opts.phases ? qpasses[pi].ntiles : passes[pi].ntiles;
+      for (uint64_t t = 0; t < nt; ++t) items.push_back(TItem{pi, 0, t});
+      L.tmax = std::max(L.tmax, opts.phases ? int(kQTileBits) : passes[pi].t);
     }
     L.nitems = items.size() - L.item_begin;
     large.push_back(L);
@@ -183,6 +221,14 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
   };
   small.grid = grid_for(small);
   for (auto& L : large) L.grid = grid_for(L);
+  if (opts.phases) {
+    // register-phase levels: as many blocks of 256 threads as the LDS holds
+    // per CU (a 2^12-amplitude tile and the ticket: 4 of complex<float>, 2 of
+    // complex<double>)
+    const uint64_t per_cu =
+        std::max<uint64_t>(1, (160u * 1024u) / ((uint64_t(1) << kQTileBits) * eb + 256u));
+    for (auto& L : large) L.grid = unsigned(std::max<uint64_t>(1, std::min<uint64_t>(L.nitems, uint64_t(cus()) * per_cu)));
+  }
   QSV_TRY(upload(d_circs, circs));
   QSV_TRY(upload(d_passes, passes));
   QSV_TRY(upload(d_ops, ops.empty() ? std::vector<TOp>(1) : ops));
@@ -190,6 +236,12 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
   QSV_TRY(upload(d_tickets, std::vector<unsigned long long>(1 + large.size(), 0)));
   QSV_TRY(upload(d_done, std::vector<unsigned int>(std::max(1, ncirc), 0)));
   QSV_TRY(upload(d_stamps, std::vector<unsigned long long>(ncirc + 1, 0)));
+  if (opts.phases && !qpasses.empty()) {
+    QSV_TRY(upload(d_qpasses, qpasses));
+    QSV_TRY(upload(d_qphases, qphases));
+    QSV_TRY(upload(d_qops, qops.empty() ? std::vector<QOp>(1) : qops));
+    QSV_TRY(upload(d_qpinfo, qpinfo));
+  }
   return QSV_OK;
 }
 
@@ -219,9 +271,30 @@ int Batch::enqueue(void* psi, hipStream_t st) {
   QSV_CHECK_HIP(hipStreamWaitEvent(side, fork, 0));
   if (small.nitems)
     QSV_TRY(dbl ? launch_tile_level<double>(args(small, 0), side) : launch_tile_level<float>(args(small, 0), side));
-  for (size_t k = 0; k < large.size(); ++k)
-    QSV_TRY(dbl ? launch_tile_level<double>(args(large[k], int(1 + k)), st)
-                : launch_tile_level<float>(args(large[k], int(1 + k)), st));
+  auto qargs = [&](const Level& L, int ticket) {
+    QTileLevelArgs a;
+    a.psi = psi;
+    a.circs = static_cast<const TCircuit*>(d_circs.p);
+    a.passes = static_cast<const QPass*>(d_qpasses.p);
+    a.pinfo = static_cast<const QPassInfo*>(d_qpinfo.p);
+    a.phases = static_cast<const QPhase*>(d_qphases.p);
+    a.ops = static_cast<const QOp*>(d_qops.p);
+    a.items = static_cast<const TItem*>(d_items.p) + L.item_begin;
+    a.nitems = L.nitems;
+    a.ticket = tickets + ticket;
+    a.done_tiles = static_cast<unsigned int*>(d_done.p);
+    a.stamps = stamps;
+    a.grid = L.grid;
+    return a;
+  };
+  for (size_t k = 0; k < large.size(); ++k) {
+    if (opts.phases)
+      QSV_TRY(dbl ? launch_qtile_level<double>(qargs(large[k], int(1 + k)), st)
+                  : launch_qtile_level<float>(qargs(large[k], int(1 + k)), st));
+    else
+      QSV_TRY(dbl ? launch_tile_level<double>(args(large[k], int(1 + k)), st)
+                  : launch_tile_level<float>(args(large[k], int(1 + k)), st));
+  }
   QSV_CHECK_HIP(hipEventRecord(join, side));
   QSV_CHECK_HIP(hipStreamWaitEvent(st, join, 0));
   return QSV_OK;
@@ -326,17 +399,16 @@ int qsv_batch_info(const qsv_batch* b, qsv_batch_info_t* info, uint64_t* offsets
     std::memset(info, 0, sizeof(*info));
     info->ncirc = B.ncirc;
     info->total_amplitudes = B.offsets.back();
-    info->npasses = int64_t(B.passes.size());
-    info->nops = int64_t(B.ops.size());
+    info->npasses = int64_t(B.passes.size() + B.qpasses.size());
+    info->nops = int64_t(B.ops.size() + B.qops.size());
     info->nlevels = int64_t(B.large.size()) + (B.small.nitems ? 1 : 0);
     info->opts = B.opts;
-    int64_t s = 0;
-    for (int c = 0; c < B.ncirc; ++c) s += B.nq[c] <= B.opts.small_bits;
-    info->nsmall = s;
+    info->nsmall = int64_t(B.small.nitems);  // one item per circuit run whole in LDS
     const double eb = B.dtype == QSV_C128 ? 16.0 : 8.0;
     double bytes = 0;
     for (const TPass& P : B.passes)
       bytes += (P.init_zero ? 1.0 : 2.0) * std::ldexp(eb, B.nq[size_t(P.circuit)]);
+    for (const QPassInfo& q : B.qpinfo) bytes += (q.first ? 1.0 : 2.0) * std::ldexp(eb, B.nq[size_t(q.circuit)]);
     info->alg_bytes = bytes;
   }
   return QSV_OK;
@@ -414,6 +486,17 @@ int qsv_batch_autotune(int dtype, int ncirc, const int* nqubits, const qsv_gate*
       o.tile_bits = t;
       o.small_bits = cap;
       o.fuse_k = k;
+      cands.push_back(o);
+    }
+  // register-phase passes for the larger circuits, with the circuits up to
+  // cap or up to 11 qubits whole in LDS (fusion width of those: 1 or 2)
+  for (int sb : {cap, kQTileBits - 1})
+    for (int k = 1; k <= 2; ++k) {
+      qsv_batch_opts o{};
+      o.tile_bits = kQTileBits;
+      o.small_bits = sb;
+      o.fuse_k = k;
+      o.phases = 1;
       cands.push_back(o);
     }
   hipEvent_t e0, e1;

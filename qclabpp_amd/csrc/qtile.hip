@@ -2,6 +2,7 @@
 // 2^kQTileBits amplitudes and applies a pass's gates phase by phase, each
 // phase in registers (16 amplitudes per thread over the phase's 4 bits).
 #include "qtile.hpp"
+#include "tile_launch.hpp"
 
 namespace qsv {
 
@@ -376,6 +377,120 @@ __global__ __launch_bounds__(kQThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   }
 }
 
+// One dependency level of the batch engine's register-phase circuits: the
+// passes of k_qtile over many programs.  Persistent blocks take (pass, tile)
+// work items from a ticket counter; an item loads its tile (a circuit's first
+// pass sets |0...0> instead: the amplitude 1 sits at position 0 of tile 0),
+// runs the pass's phases as k_qtile does, and stores the tile.  The block
+// that stores the last tile of a circuit's last pass stamps its completion.
+// Items of one level never depend on each other.  Each block takes its next
+// ticket when it starts an item and loads that item's amplitudes into
+// registers while the current item's phases run.
+template <typename R>
+__global__ __launch_bounds__(kQThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_qtile_level(
+    typename CT<R>::type* __restrict__ psi, const TCircuit* __restrict__ circs, QPassPtr passes,
+    const QPassInfo* __restrict__ pinfo, QPhasePtr phases, QOpPtr ops, const TItem* __restrict__ items,
+    uint64_t nitems, unsigned long long* ticket, unsigned int* done_tiles, unsigned long long* stamps) {
+  using C = typename CT<R>::type;
+  __shared__ C tile[TE];
+  __shared__ unsigned long long next_ticket;
+  const int tid = threadIdx.x;
+  const uint32_t tslot = qtile_swizzle(uint32_t(tid));
+  // the ticket read from LDS is the same in every lane: made scalar, the
+  // item, pass and phase descriptors are read with scalar loads
+  auto uniform = [](unsigned long long x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+    return (uint64_t(hi) << 32) | lo;
+  };
+  // global offset of this thread's tile position tid + 256*j for an item:
+  // the tile's base and the thread's low 8 bits (lo), then j's 4 bits
+  struct Where {
+    uint64_t lo;
+    uint64_t hb[4];
+    __device__ uint64_t at(int j) const {
+      uint64_t x = lo;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (j >> i & 1) x |= hb[i];
+      return x;
+    }
+  };
+  auto where = [&](const TItem& item) {
+    const QCONST QPass& pass = passes[item.pass];
+    Where w;
+    uint64_t o = 0;
+    int sh = 0;
+    for (int r = 0; r < pass.nruns; ++r) {
+      const int len = pass.rlen[r];
+      o |= ((item.tile >> sh) & ((uint64_t(1) << len) - 1)) << pass.rdst[r];
+      sh += len;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (tid >> i & 1) o |= uint64_t(1) << pass.tbits[i];
+    w.lo = o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w.hb[i] = uint64_t(1) << pass.tbits[8 + i];
+    return w;
+  };
+  auto load = [&](uint64_t it, C (&pre)[NR]) {
+    if (it >= nitems) return;
+    const TItem item = items[it];
+    const QPassInfo pi = pinfo[item.pass];
+    if (pi.first) return;
+    const C* state = psi + circs[pi.circuit].offset;
+    const Where w = where(item);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) pre[j] = ld_stream(&state[w.at(j)]);
+  };
+  if (tid == 0) next_ticket = atomicAdd(ticket, 1ull);
+  __syncthreads();
+  uint64_t it = uniform(next_ticket);
+  C pre[NR];
+  load(it, pre);
+  while (it < nitems) {
+    __syncthreads();  // every lane has read next_ticket
+    if (tid == 0) next_ticket = atomicAdd(ticket, 1ull);
+    const TItem item = items[it];
+    const QCONST QPass& pass = passes[item.pass];
+    const QPassInfo pi = pinfo[item.pass];
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+      tile[tslot ^ qtile_swizzle(uint32_t(kQThreads * j))] =
+          pi.first ? mk<C>(item.tile == 0 && tid == 0 && j == 0 ? 1 : 0, 0) : pre[j];
+    __syncthreads();
+    const uint64_t itn = uniform(next_ticket);
+    load(itn, pre);
+    for (int ph = pass.phase_begin; ph < pass.phase_end; ++ph) {
+      const QCONST QPhase& P = phases[ph];
+      const uint32_t tp = qtile_swizzle(pdep12(uint32_t(tid), ~P.regmask & uint32_t(TE - 1)));
+      C v[NR];
+#pragma unroll
+      for (int i = 0; i < NR; ++i) v[i] = tile[tp ^ P.regpos[i]];
+      const int ob = P.op_begin, oe = P.op_end;
+      for (int o = ob; o < oe; ++o) apply_op(ops[o], v);
+#pragma unroll
+      for (int i = 0; i < NR; ++i) tile[tp ^ P.regpos_out[i]] = v[i];
+      __syncthreads();
+    }
+    C* state = psi + circs[pi.circuit].offset;
+    const Where w = where(item);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) st_stream(&state[w.at(j)], tile[tslot ^ qtile_swizzle(uint32_t(kQThreads * j))]);
+    if (pi.last) {
+      __threadfence();
+      __syncthreads();
+      if (tid == 0) {
+        const unsigned int prev = atomicAdd(&done_tiles[pi.circuit], 1u);
+        if (uint64_t(prev) + 1 == pass.ntiles) stamps[pi.circuit] = wall_clock64();
+      }
+    }
+    __syncthreads();  // the tile is free again before the next item
+    it = itn;
+  }
+}
+
 }  // namespace
 
 // Blocks per CU in the grid: 2 are resident at a time (two waves per SIMD,
@@ -411,5 +526,18 @@ int launch_qtile(const QTileLaunch& a, hipStream_t st) {
 
 template int launch_qtile<double>(const QTileLaunch&, hipStream_t);
 template int launch_qtile<float>(const QTileLaunch&, hipStream_t);
+
+template <typename R>
+int launch_qtile_level(const QTileLevelArgs& a, hipStream_t st) {
+  using C = typename CT<R>::type;
+  hipLaunchKernelGGL((k_qtile_level<R>), dim3(a.grid), dim3(kQThreads), 0, st, static_cast<C*>(a.psi), a.circs,
+                     (QPassPtr)a.passes, a.pinfo, (QPhasePtr)a.phases, (QOpPtr)a.ops, a.items, a.nitems, a.ticket,
+                     a.done_tiles, a.stamps);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+template int launch_qtile_level<double>(const QTileLevelArgs&, hipStream_t);
+template int launch_qtile_level<float>(const QTileLevelArgs&, hipStream_t);
 
 }  // namespace qsv

@@ -65,10 +65,32 @@ def test_mixed_batch_vs_oracle(dtype, tol, fuse_k, tile_bits, small_bits):
     check_all(b, packed, circuits, nq, tol)
 
 
-def test_graph_replays_match_eager():
+@pytest.mark.parametrize("dtype,tol", [("c128", C128), ("c64", C64)])
+@pytest.mark.parametrize("fuse_k", [1, 2])
+@pytest.mark.parametrize("small_bits", [6, 11, 13])
+def test_register_phase_batch_vs_oracle(dtype, tol, fuse_k, small_bits):
+    """phases=1: circuits above max(small_bits, 11) qubits run register-phase
+    passes of 2^12-amplitude tiles (first pass from |0...0> without a read),
+    the rest whole in LDS; the circuits of 12 to 15 qubits take one to
+    several passes each."""
+    print(f"batch phases {dtype} fuse_k={fuse_k} small={small_bits}", flush=True)
+    nq = [1, 2, 3, 5, 7, 9, 11, 12, 14, 15, 8, 6, 13, 4, 10, 16]
+    circuits = [mixed_circuit(n, 31 + i) for i, n in enumerate(nq)]
+    b = B.Batch(circuits, nq, dtype=dtype, small_bits=small_bits, fuse_k=fuse_k, phases=1)
+    assert b.info["phases"] == 1
+    assert b.info["nsmall"] == sum(1 for n in nq if n <= max(small_bits, 11))
+    packed = b.new_buffer()
+    packed.fill_(float("nan"))               # every amplitude is written
+    b.run(packed, use_graph=False)
+    check_all(b, packed, circuits, nq, tol)
+
+
+@pytest.mark.parametrize("phases", [0, 1])
+def test_graph_replays_match_eager(phases):
     import torch
     spec = W.hea_batch(48, seed=7, nmin=4, nmax=17, lmin=2, lmax=6)
-    b = B.Batch(spec.circuits, spec.nq, dtype="c64", tile_bits=10, small_bits=12, fuse_k=1)
+    b = B.Batch(spec.circuits, spec.nq, dtype="c64", tile_bits=10, small_bits=12, fuse_k=1,
+                phases=phases)
     eager = b.new_buffer()
     b.run(eager, use_graph=False)
     torch.cuda.synchronize()
@@ -92,7 +114,10 @@ def test_autotuner_candidates_agree():
     packed = torch.empty(total, dtype=torch.complex128, device="cuda")
     scratch = torch.empty_like(packed)
     best, results = B.autotune(spec.circuits, nq, packed, scratch, dtype="c128", reps=1)
-    assert len(results) == 12  # tile bits 10..13 (complex128 caps at 13) x fuse_k 0..2
+    # tile bits 10..13 (complex128 caps at 13) x fuse_k 0..2, then register-phase
+    # passes: small_bits 13 and 11 x fuse_k 1..2
+    assert len(results) == 16
+    assert sum(r["phases"] for r in results) == 4
     assert max(r["max_abs_diff"] for r in results) < 1e-12
     b = B.Batch(spec.circuits, nq, dtype="c128", **best)
     out = b.new_buffer()
