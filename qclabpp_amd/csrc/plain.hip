@@ -40,29 +40,69 @@ struct M4 {
 };
 
 // ---- apply2: 2x2 on pairs (a, a|s) -----------------------------------------
+// Pair sweeps with the partner 2^b >= 64 amplitudes away: lanes 32-63 of a
+// wave hold the amplitudes 2^b above those of lanes 0-31, so each load
+// instruction reads 32 |0>-side and 32 |1>-side amplitudes, and
+// v_permlane32_swap gives every lane both amplitudes of its pair.  Against
+// two loads per lane (one per side), over the rotations of an n=30 complex128
+// brickwork: 0.75-0.76 of HBM peak instead of 0.74.  NP pairs per lane pair:
+// 2 where the two sides lie 2^12..2^21 amplitudes apart (64 KiB..32 MiB of
+// complex128), 1 elsewhere, the faster of the two at most b on that state.
+__device__ __forceinline__ void swap_halves(uint32_t v, uint32_t& lo, uint32_t& hi) {
+  // lo: every lane gets v of lane & 31; hi: v of lane | 32
+  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  lo = r[0];
+  hi = r[1];
+}
+template <typename C>
+__device__ __forceinline__ void pair_across_halves(const C v, C& x, C& y) {
+  using R = decltype(C::x);
+  // a copy of each component: __builtin_bit_cast of a vector element
+  // expression reads element 0
+  const R re = v.x, im = v.y;
+  if constexpr (sizeof(R) == 8) {
+    const uint64_t br = __builtin_bit_cast(uint64_t, re), bi = __builtin_bit_cast(uint64_t, im);
+    uint32_t l0, h0, l1, h1, l2, h2, l3, h3;
+    swap_halves(uint32_t(br), l0, h0);
+    swap_halves(uint32_t(br >> 32), l1, h1);
+    swap_halves(uint32_t(bi), l2, h2);
+    swap_halves(uint32_t(bi >> 32), l3, h3);
+    x = mk<C>(__builtin_bit_cast(double, (uint64_t(l1) << 32) | l0), __builtin_bit_cast(double, (uint64_t(l3) << 32) | l2));
+    y = mk<C>(__builtin_bit_cast(double, (uint64_t(h1) << 32) | h0), __builtin_bit_cast(double, (uint64_t(h3) << 32) | h2));
+  } else {
+    uint32_t l0, h0, l1, h1;
+    swap_halves(__builtin_bit_cast(uint32_t, re), l0, h0);
+    swap_halves(__builtin_bit_cast(uint32_t, im), l1, h1);
+    x = mk<C>(__builtin_bit_cast(float, l0), __builtin_bit_cast(float, l1));
+    y = mk<C>(__builtin_bit_cast(float, h0), __builtin_bit_cast(float, h1));
+  }
+}
+
+inline int pair_np(int b) { return b >= 12 && b <= 21 ? 2 : 1; }
+
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_dense1(C* __restrict__ psi, uint64_t cnt, int b,
                                                 M2<C> m, uint64_t blk0) {
-  const uint64_t s = uint64_t(1) << b;
-  const uint64_t k0 = (blk0 + blockIdx.x) * (TPB * NP) + threadIdx.x;
-  C x[NP], y[NP];
+  const int lane = threadIdx.x & 63, i = lane & 31;
+  const bool upper = lane >= 32;
+  const uint64_t wave = (blk0 + blockIdx.x) * (TPB / 64) + (threadIdx.x >> 6);
+  const uint64_t kb = wave * (32 * NP) + i;
+  const uint64_t side = upper ? uint64_t(1) << b : 0;
+  C v[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
-    const uint64_t k = k0 + uint64_t(j) * TPB;
-    if (k < cnt) {
-      const uint64_t a = ins0(k, b);
-      x[j] = ld_stream(&psi[a]);
-      y[j] = ld_stream(&psi[a | s]);
-    }
+    const uint64_t k = kb + uint64_t(j) * 32;
+    v[j] = mk<C>(0, 0);
+    if (k < cnt) v[j] = ld_stream(&psi[ins0(k, b) | side]);
   }
+  // the lane's row of the 2x2: [a00 a01] for the |0> side, [a10 a11] for |1>
+  const C r0 = upper ? m.a10 : m.a00, r1 = upper ? m.a11 : m.a01;
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
-    const uint64_t k = k0 + uint64_t(j) * TPB;
-    if (k < cnt) {
-      const uint64_t a = ins0(k, b);
-      st_stream(&psi[a], cmad(m.a01, y[j], cmul(m.a00, x[j])));
-      st_stream(&psi[a | s], cmad(m.a11, y[j], cmul(m.a10, x[j])));
-    }
+    const uint64_t k = kb + uint64_t(j) * 32;
+    C x, y;
+    pair_across_halves(v[j], x, y);
+    if (k < cnt) st_stream(&psi[ins0(k, b) | side], cmad(r1, y, cmul(r0, x)));
   }
 }
 
@@ -126,42 +166,9 @@ __global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cn
 }
 
 // ---- apply4 (controlled): 2x2 on the control==cs half ------------------------
-// Lanes 32-63 of a wave hold the amplitudes 2^bt above those of lanes 0-31,
-// so each load instruction reads 32 |0>-side and 32 |1>-side amplitudes;
-// v_permlane32_swap then gives every lane both amplitudes of its pair.
-// Against two loads per lane of the two sides: 0.72 of HBM peak instead of
-// 0.69 over neighbour-qubit CX gates at n=30 complex128 (the control half
-// of the state, read and written in place).
-__device__ __forceinline__ void swap_halves(uint32_t v, uint32_t& lo, uint32_t& hi) {
-  // lo: every lane gets v of lane & 31; hi: v of lane | 32
-  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-  lo = r[0];
-  hi = r[1];
-}
-template <typename C>
-__device__ __forceinline__ void pair_across_halves(const C v, C& x, C& y) {
-  using R = decltype(C::x);
-  // a copy of each component: __builtin_bit_cast of a vector element
-  // expression reads element 0
-  const R re = v.x, im = v.y;
-  if constexpr (sizeof(R) == 8) {
-    const uint64_t br = __builtin_bit_cast(uint64_t, re), bi = __builtin_bit_cast(uint64_t, im);
-    uint32_t l0, h0, l1, h1, l2, h2, l3, h3;
-    swap_halves(uint32_t(br), l0, h0);
-    swap_halves(uint32_t(br >> 32), l1, h1);
-    swap_halves(uint32_t(bi), l2, h2);
-    swap_halves(uint32_t(bi >> 32), l3, h3);
-    x = mk<C>(__builtin_bit_cast(double, (uint64_t(l1) << 32) | l0), __builtin_bit_cast(double, (uint64_t(l3) << 32) | l2));
-    y = mk<C>(__builtin_bit_cast(double, (uint64_t(h1) << 32) | h0), __builtin_bit_cast(double, (uint64_t(h3) << 32) | h2));
-  } else {
-    uint32_t l0, h0, l1, h1;
-    swap_halves(__builtin_bit_cast(uint32_t, re), l0, h0);
-    swap_halves(__builtin_bit_cast(uint32_t, im), l1, h1);
-    x = mk<C>(__builtin_bit_cast(float, l0), __builtin_bit_cast(float, l1));
-    y = mk<C>(__builtin_bit_cast(float, h0), __builtin_bit_cast(float, h1));
-  }
-}
-
+// The pair layout of k_dense1 over the control half (0.73-0.74 of HBM peak
+// against 0.69 with two loads per lane, over neighbour-qubit CX gates at n=30
+// complex128, the control half read and written in place).
 // blo < bhi are the two bit positions; a1 carries the control bit, s the
 // target.  Each wave takes 32*NP pairs; lanes whose pair is out of range
 // still take part in the swaps (both lanes of a pair are in or out together).
@@ -433,7 +440,7 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
   const uint64_t N = uint64_t(1) << n;
   // items per thread, measured on gfx950 at n=30: one for the full and half
   // sweeps (more waves in flight), two for the controlled quarter-pair sweep
-  constexpr int NP = 1, NPC = 2;
+  constexpr int NP = 1;
   const int per = TPB * NP;
   // address bits inside one 128-byte line: a sweep that touches only some
   // amplitudes of a line is replaced by one that reads and writes whole lines
@@ -449,8 +456,10 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
         QSV_LAUNCH_BLOCKS((k_dense1_lanes<C, NP>), blocks_for(N, per), TPB, 0, st,
                            psi, N, b, m);
       } else {
-        QSV_LAUNCH_BLOCKS((k_dense1<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi,
-                           N / 2, b, m);
+        if (pair_np(b) == 2)
+          QSV_LAUNCH_BLOCKS((k_dense1<C, 2>), blocks_for(N / 2, TPB), TPB, 0, st, psi, N / 2, b, m);
+        else
+          QSV_LAUNCH_BLOCKS((k_dense1<C, 1>), blocks_for(N / 2, TPB / 2), TPB, 0, st, psi, N / 2, b, m);
       }
       break;
     }
@@ -485,11 +494,15 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
         QSV_LAUNCH_BLOCKS((k_cdense1_full<C, NP>), blocks_for(N / 2, per), TPB, 0, st, psi, N / 2,
                           bt, bc, op.cs, m);
       } else {
-        // 2 pairs per lane pair: 32*2 pairs per wave, TPB/64 waves per block
+        // NP pairs per lane pair: 32*NP pairs per wave, TPB/64 waves per block
         const uint64_t a1 = uint64_t(op.cs) << bc;
-        QSV_LAUNCH_BLOCKS((k_cdense1<C, NPC>), blocks_for(N / 4, (TPB / 2) * NPC), TPB, 0, st,
-                          psi, N / 4, std::min(bc, bt), std::max(bc, bt), a1,
-                          uint64_t(1) << bt, m);
+        const int blo = std::min(bc, bt), bhi = std::max(bc, bt);
+        if (pair_np(bt) == 2)
+          QSV_LAUNCH_BLOCKS((k_cdense1<C, 2>), blocks_for(N / 4, TPB), TPB, 0, st, psi, N / 4, blo,
+                            bhi, a1, uint64_t(1) << bt, m);
+        else
+          QSV_LAUNCH_BLOCKS((k_cdense1<C, 1>), blocks_for(N / 4, TPB / 2), TPB, 0, st, psi, N / 4,
+                            blo, bhi, a1, uint64_t(1) << bt, m);
       }
       break;
     }
