@@ -215,6 +215,10 @@ typedef struct qsv_program_info {
                           (FMA and multiply; FP64 for complex128, FP32 for
                           complex64) of one run's gate arithmetic, as the
                           compiled passes generate it                    */
+  double  work_bytes;  /* compiled tiled programs whose relabeling ends out
+                          of place: device bytes of the program's work
+                          buffer (one state); 0 otherwise.  Runs of such a
+                          program on different streams must not overlap */
 } qsv_program_info;
 
 int qsv_program_create(int n, int dtype, const qsv_gate* gates, size_t ngates, int fuse_k,

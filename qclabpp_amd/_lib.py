@@ -107,6 +107,7 @@ class QsvProgramInfo(C.Structure):
         ("ntiled_phases", C.c_int64),
         ("ntiled_relabel", C.c_int64),
         ("fp_lane_ops", C.c_double),
+        ("work_bytes", C.c_double),
     ]
 
 

@@ -239,8 +239,9 @@ int qsv_tiled_jit_compile_check_form(int n, int dtype, const qsv_gate* gates, si
   std::vector<KOp> lowered(ngates);
   for (size_t i = 0; i < ngates; ++i) QSV_TRY(lower_gate(gates[i], n, lowered[i]));
   QProgram qp;
+  // as Program::build does on a device with room for the work buffer
   QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qp,
-                       jit_tile_bits(dtype, n, fuse_k == QSV_TILED_JIT_FIXED), true));
+                       jit_tile_bits(dtype, n, fuse_k == QSV_TILED_JIT_FIXED), true, true));
   qtile_scale_rotations(qp);  // as Program::build does
   qp.fixed_forms = fuse_k == QSV_TILED_JIT_FIXED;
   size_t total = 0;

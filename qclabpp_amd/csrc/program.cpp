@@ -1,5 +1,6 @@
 // Compiled circuits: lowering, optional fusion, device coefficient storage,
 // and the launch sequence of one run.
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <memory>
@@ -86,7 +87,33 @@ int Program::build(int n_, int dtype_, const qsv_gate* gates, size_t ngates, int
     // fixed-form complex128 passes, 2^12 for complex128 passes that branch
     // per rotation)
     const int tb = jit ? jit_tile_bits(dtype, n, fixed_forms) : kQTileBits;
-    QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt, tb, jit));
+    // A relabeling schedule may end out of place (no gate-free restore
+    // pass) when a work buffer of the state's size can be held: allocated
+    // here when the device has room for it and for the caller's state
+    // besides (twice the state bytes and 2 GiB free), freed again when the
+    // schedule does not use it.  QSV_JIT_OUT_OF_PLACE=0 turns it off.
+    bool oop = false;
+    if (jit) {
+      const char* e = std::getenv("QSV_JIT_OUT_OF_PLACE");
+      const size_t bytes = (size_t(1) << n) * (dtype == QSV_C128 ? 16 : 8);
+      size_t free_b = 0, total_b = 0;
+      if (!(e && e[0] == '0') && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+          free_b >= 2 * bytes + (size_t(2) << 30)) {
+        if (hipMalloc(&work, bytes) == hipSuccess) {
+          work_bytes = bytes;
+          oop = true;
+        } else {
+          work = nullptr;
+        }
+      }
+      (void)hipGetLastError();
+    }
+    QSV_TRY(compile_qtile(lowered, n, dtype == QSV_C128 ? 3 : 4, qt, tb, jit, oop));
+    if (work && !qtile_uses_work(qt)) {
+      (void)hipFree(work);
+      work = nullptr;
+      work_bytes = 0;
+    }
     qtile_scale_rotations(qt);
     qt.fixed_forms = fixed_forms;
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
@@ -159,6 +186,7 @@ int Program::upload() const {
 
 Program::~Program() {
   if (dbuf) (void)hipFree(dbuf);
+  if (work) (void)hipFree(work);
 }
 
 int Program::run(void* psi, hipStream_t st) const {
@@ -174,7 +202,7 @@ int Program::run(void* psi, hipStream_t st) const {
     a.ops = reinterpret_cast<const QOp*>(base + qt_op_off);
     if (jit) {
       for (size_t i = 0; i < qt.passes.size(); ++i)
-        QSV_TRY(qtile_jit_launch(qj, i, psi, a.ops, st));
+        QSV_TRY(qtile_jit_launch(qj, i, psi, work, a.ops, st));
       return QSV_OK;
     }
     for (size_t i = 0; i < qt.passes.size(); ++i) {
@@ -206,6 +234,7 @@ void Program::info(qsv_program_info* out) const {
       out->ntiled_relabel += std::memcmp(P.tbits, P.tbits_out, sizeof(P.tbits)) != 0 ? 1 : 0;
     out->alg_bytes = double(qt.passes.size()) * 2.0 * std::ldexp(1.0, n) * eb;
     out->fp_lane_ops = qtile_fp_lane_ops(qt);
+    out->work_bytes = double(work_bytes);
     return;
   }
   out->nlaunch = int64_t(ops.size());

@@ -32,6 +32,11 @@ struct Program {
   QJit qj;
   size_t qt_pass_off = 0, qt_phase_off = 0, qt_op_off = 0;
   mutable void* dbuf = nullptr;
+  // work buffer of the state's size for a schedule with an out-of-place end
+  // (the last two passes: state -> work -> state); runs of one such program
+  // on different streams must not overlap
+  void* work = nullptr;
+  size_t work_bytes = 0;
   mutable bool uploaded = false;
   mutable std::mutex mu;
 

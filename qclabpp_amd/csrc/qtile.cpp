@@ -273,7 +273,7 @@ namespace {
 // absorb equally many pending gates: strategy 0 takes the highest address
 // bit, 1 the lowest, others a pseudo-random one (seeded by the strategy).
 int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, int strategy,
-                       bool relabel, QProgram& out) {
+                       bool relabel, bool oop, QProgram& out) {
   constexpr int RB = kQRegBits, NR = 1 << kQRegBits;
   uint64_t rng = 0x9E3779B97F4A7C15ull * uint64_t(strategy + 1);
   auto next = [&]() {
@@ -420,6 +420,45 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
         for (size_t i = 0; i < addr.size() && inS[q] && !done[q]; ++i)
           if (addr[i] >= line_bits && !used[addr[i]]) put(q, addr[i]);
     }
+    // ---- an out-of-place end: when this pass takes every remaining gate and
+    // would leave qubits away from home, it reads the work buffer (which the
+    // pass before it writes) and writes the state with every qubit home.
+    // Its tile must hold the qubits of the home line bits (the lines it
+    // writes); they replace tile qubits that no gate of the pass touches.
+    // As for any pass, at most kLaneBits - L of them may come from off the
+    // line, so that the lines read and written fit the lanes of a wave.
+    bool oop_end = false;
+    if (relabel && oop && deferred.empty() && !restore && !out.passes.empty()) {
+      bool home = true;
+      for (int q = 0; q < n; ++q) home = home && lay2[q] == n - 1 - q;
+      int off_line = 0;
+      for (int j = 0; j < line_bits; ++j) off_line += lay[n - 1 - j] >= line_bits ? 1 : 0;
+      if (!home && off_line <= kLaneBits - line_bits) {
+        std::vector<char> keep(n, 0);
+        for (const KOp& op : mine)
+          for (int j = 0; j < op.nq; ++j) keep[op.q[j]] = 1;
+        for (int j = 0; j < line_bits; ++j) keep[at[j]] = keep[n - 1 - j] = 1;
+        std::vector<char> ins2 = inS;
+        bool ok = true;
+        for (int j = 0; j < line_bits && ok; ++j) {
+          const int h = n - 1 - j;
+          if (ins2[h]) continue;
+          int r = -1;
+          for (int q = 0; q < n && r < 0; ++q)
+            if (ins2[q] && !keep[q]) r = q;
+          if (r < 0) ok = false;
+          else {
+            ins2[r] = 0;
+            ins2[h] = 1;
+          }
+        }
+        if (ok) {
+          inS = ins2;
+          for (int q = 0; q < n; ++q) lay2[q] = n - 1 - q;
+          oop_end = true;
+        }
+      }
+    }
     // ---- pass descriptor; tile-local bits: the line bits read, then the
     // line bits written, then the rest by address
     QPass P{};
@@ -441,9 +480,11 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
       tmask |= uint64_t(1) << lay[lq[i]];
       tmask_out |= uint64_t(1) << lay2[lq[i]];
     }
-    // the pass writes exactly the addresses it reads (in place), and the
-    // lanes of a wave cover whole lines on both sides
-    if (tmask != tmask_out || __builtin_popcountll(tmask) != T)
+    // the pass writes exactly the addresses it reads (in place, unless it is
+    // an out-of-place end), and the lanes of a wave cover whole lines on
+    // both sides
+    if ((!oop_end && tmask != tmask_out) || __builtin_popcountll(tmask) != T ||
+        __builtin_popcountll(tmask_out) != T)
       return fail(QSV_EINVAL, "tiled compile: a pass would write outside its tile");
     for (int i = 0; i < T; ++i)
       if ((i < line_bits && P.tbits[i] != i) || (P.tbits_out[i] < line_bits && i >= kLaneBits))
@@ -460,6 +501,27 @@ int compile_qtile_with(const std::vector<KOp>& in, int n, int line_bits, int T, 
       P.rlen[P.nruns] = uint8_t(e - b);
       ++P.nruns;
       b = e;
+    }
+    // the tile index runs on the written side: tile index bit k (the k-th
+    // address bit outside the tile, ascending) goes where its qubit is written
+    {
+      std::vector<int> dst;
+      for (int b = 0; b < n; ++b)
+        if (!(tmask >> b & 1)) dst.push_back(lay2[at[b]]);
+      P.nruns_out = 0;
+      for (size_t k = 0; k < dst.size();) {
+        size_t e = k + 1;
+        while (e < dst.size() && dst[e] == dst[e - 1] + 1) ++e;
+        if (P.nruns_out >= sizeof(P.rdst_out)) return fail(QSV_EINVAL, "tiled compile: too many output runs");
+        P.rdst_out[P.nruns_out] = uint8_t(dst[k]);
+        P.rlen_out[P.nruns_out] = uint8_t(e - k);
+        ++P.nruns_out;
+        k = e;
+      }
+    }
+    if (oop_end) {
+      P.wsrc = 1;
+      out.passes.back().wdst = 1;
     }
     P.ntiles = uint64_t(1) << (n - T);
     auto local = [&](int q) { return loc[q]; };  // tile-local bit of qubit q
@@ -845,7 +907,7 @@ constexpr size_t kSchedCacheMax = 1024;
 std::atomic<int64_t> g_sched_hits{0}, g_sched_misses{0};
 
 std::string schedule_key(const std::vector<KOp>& in, int n, int line_bits, int tile_bits,
-                         bool relabel) {
+                         bool relabel, bool oop) {
   std::string k;
   k.reserve(16 + in.size() * 8);
   auto put = [&](int v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
@@ -853,6 +915,7 @@ std::string schedule_key(const std::vector<KOp>& in, int n, int line_bits, int t
   put(line_bits);
   put(tile_bits);
   put(relabel);
+  put(oop);
   for (const KOp& op : in) {
     put(op.kind | op.nq << 8 | (op.cs & 1) << 12 | (op.grow >= 0) << 13);
     for (int i = 0; i < op.nq; ++i) k.push_back(char(op.q[i]));
@@ -867,8 +930,14 @@ void qtile_schedule_cache_stats(int64_t* hits, int64_t* misses) {
   if (misses) *misses = g_sched_misses.load();
 }
 
+bool qtile_uses_work(const QProgram& qp) {
+  for (const QPass& P : qp.passes)
+    if (P.wsrc || P.wdst) return true;
+  return false;
+}
+
 int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out, int tile_bits,
-                  bool relabel) {
+                  bool relabel, bool out_of_place) {
   if (tile_bits < kQTileBits || tile_bits > kQTileBitsMax)
     return fail(QSV_EINVAL, "tile bits must be in [" + std::to_string(kQTileBits) + ", " +
                                 std::to_string(kQTileBitsMax) + "]");
@@ -891,7 +960,7 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
   const int tries = int(std::clamp<size_t>(size_t(128) * 1000 / gates, 8, 128));
   const int variants = relabel ? 2 : 1;
   const int total = tries * variants;
-  const std::string key = schedule_key(in, n, line_bits, tile_bits, relabel);
+  const std::string key = schedule_key(in, n, line_bits, tile_bits, relabel, out_of_place);
   int known = -1;
   {
     std::lock_guard<std::mutex> lk(g_sched_mu);
@@ -900,7 +969,8 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
   }
   if (known >= 0 && known < total) {
     QProgram p;
-    if (compile_qtile_with(in, n, line_bits, tile_bits, known % tries, known >= tries, p) == QSV_OK) {
+    if (compile_qtile_with(in, n, line_bits, tile_bits, known % tries, known >= tries, out_of_place, p) ==
+        QSV_OK) {
       ++g_sched_hits;
       out = std::move(p);
       out.tile_bits = tile_bits;
@@ -916,7 +986,7 @@ int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& ou
     std::atomic<int> next{0};
     auto work = [&]() {
       for (int i; (i = next.fetch_add(1)) < total;) {
-        rc[i] = compile_qtile_with(in, n, line_bits, tile_bits, i % tries, i >= tries, cand[i]);
+        rc[i] = compile_qtile_with(in, n, line_bits, tile_bits, i % tries, i >= tries, out_of_place, cand[i]);
         if (rc[i] != QSV_OK) err[i] = get_error();
       }
     };

@@ -87,13 +87,25 @@ inline __host__ __device__ uint32_t qtile_swizzle(uint32_t p) {
 // line bits written are among local bits 0..2L-1, so the lanes of a wave
 // cover whole 128-byte lines on both sides.  Without relabeling the local
 // bits are the address bits in ascending order and tbits_out == tbits.
+//
+// Out-of-place end (compiled passes only): a relabeling schedule whose last
+// gate pass would leave qubits away from their home addresses may instead
+// have its second-to-last pass write a work buffer of the state's size
+// (wdst) and its last pass read that buffer (wsrc) and write the state with
+// every qubit home: the last pass's tbits_out and its out-runs (where the
+// tile index bits go) then permute all address bits, not only its tile's,
+// and no gate-free restore pass is needed.
 struct QPass {
   int32_t phase_begin, phase_end;
   int32_t nruns;
   uint8_t tbits[16];      // address bit of tile-local bit i when read
   uint8_t tbits_out[16];  // address bit of tile-local bit i when written
-  uint8_t rdst[24];       // runs of address bits outside the tile
+  uint8_t rdst[24];       // runs of address bits outside the tile (read side)
   uint8_t rlen[24];
+  uint8_t wsrc, wdst;     // reads / writes the work buffer instead of the state
+  uint8_t nruns_out, pad_;
+  uint8_t rdst_out[24];   // the same tile index runs on the written side
+  uint8_t rlen_out[24];
   uint64_t ntiles;
 };
 
@@ -115,8 +127,13 @@ struct QProgram {
 // up to kQTileBitsMax for compiled passes).  relabel: passes may move qubits
 // onto the line bits (tbits_out != tbits; compiled passes only), so a tile
 // need not spend its line bits on qubits the next gates do not touch.
+// out_of_place: a relabeling schedule may end out of place (QPass wsrc /
+// wdst), which needs a work buffer of the state's size at run time.
 int compile_qtile(const std::vector<KOp>& in, int n, int line_bits, QProgram& out,
-                  int tile_bits = kQTileBits, bool relabel = false);
+                  int tile_bits = kQTileBits, bool relabel = false, bool out_of_place = false);
+
+// Whether a program's passes use the work buffer (an out-of-place end).
+bool qtile_uses_work(const QProgram& qp);
 
 // Rewrites the real rotations of every pass into scaled forms that cost one
 // FMA per real output component instead of two (a rotation [[c, -s], [s, c]]

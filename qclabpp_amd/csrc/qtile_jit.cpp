@@ -338,6 +338,38 @@ uint32_t line_mask(const uint8_t* tb, int T, bool c128) {
   return m;
 }
 
+// Parameters of a pass kernel: the state in place, or for a pass at an
+// out-of-place end the buffer it reads and the one it writes.
+std::string pass_params(const QPass& P) {
+  if (P.wsrc || P.wdst) return "(const C* __restrict__ pin, C* __restrict__ pout, const QC R* cf0, unsigned long long ntiles)";
+  return "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles)";
+}
+const char* src_name(const QPass& P) { return P.wsrc || P.wdst ? "pin" : "psi"; }
+const char* dst_name(const QPass& P) { return P.wsrc || P.wdst ? "pout" : "psi"; }
+bool same_out_runs(const QPass& P) {
+  if (P.nruns_out != P.nruns) return false;
+  for (int r = 0; r < P.nruns; ++r)
+    if (P.rdst_out[r] != P.rdst[r] || P.rlen_out[r] != P.rlen[r]) return false;
+  return true;
+}
+// tile_base(t): the tile's address bits outside the tile, on the read side;
+// tile_base_out(t) the same on the written side when the pass moves them
+void emit_tile_bases(std::ostringstream& s, const QPass& P) {
+  auto one = [&](const char* name, int nruns, const uint8_t* dst, const uint8_t* len) {
+    s << "  auto " << name << " = [&](unsigned long long t) {\n"
+      << "    unsigned long long b = 0;\n";
+    for (int r = 0, sh = 0; r < nruns; ++r) {
+      s << "    b |= ((t >> " << sh << ") & " << u64((uint64_t(1) << len[r]) - 1) << ") << " << int(dst[r]) << ";\n";
+      sh += len[r];
+    }
+    s << "    return b;\n"
+      << "  };\n";
+  };
+  one("tile_base", P.nruns, P.rdst, P.rlen);
+  if (!same_out_runs(P)) one("tile_base_out", P.nruns_out, P.rdst_out, P.rlen_out);
+}
+const char* out_base(const QPass& P) { return same_out_runs(P) ? "base" : "tile_base_out(t)"; }
+
 // Kernel names: qtp_<pass>, or qtf_<pass> for fixed-form passes (so that
 // profiles keep the two forms apart).
 const char* pass_prefix(const QProgram& qp) { return qp.fixed_forms ? "qtf_" : "qtp_"; }
@@ -389,18 +421,11 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
   // (64 KiB, 2 blocks of 8 waves)
   const int wpe = c128 ? (T == 12 ? 2 : 1) : (T == 12 ? 3 : 4);
   s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(" << wpe << ")))\n"
-    << "void " << pass_prefix(qp) << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
+    << "void " << pass_prefix(qp) << pi << pass_params(P) << " {\n";
   if (!(din && dout && K == 1)) s << "  __shared__ C tile[" << (1u << T) << "];\n";
-  s << "  const unsigned tid = threadIdx.x;\n"
-    << "  auto tile_base = [&](unsigned long long t) {\n"
-    << "    unsigned long long b = 0;\n";
-  for (int r = 0, sh = 0; r < P.nruns; ++r) {
-    s << "    b |= ((t >> " << sh << ") & " << u64((uint64_t(1) << P.rlen[r]) - 1) << ") << " << int(P.rdst[r])
-      << ";\n";
-    sh += P.rlen[r];
-  }
-  s << "    return b;\n"
-    << "  };\n";
+  s << "  const unsigned tid = threadIdx.x;\n";
+  emit_tile_bases(s, P);
+  const std::string src = src_name(P), dst = dst_name(P), obase = out_base(P);
   // offsets of the 16 amplitudes a thread loads (and stores) at the ends:
   // the first (last) phase's registers, or tile positions tid + threads*j
   std::vector<uint64_t> in_ofs(NR), out_ofs(NR);
@@ -430,7 +455,7 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
     << "  if (t < tend) {\n"
     << "    const unsigned long long b0 = tile_base(t) + gin;\n"
     << "#pragma unroll\n"
-    << "    for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[b0 + oin[j]]);\n"
+    << "    for (int j = 0; j < 16; ++j) pre[j] = ldnt(&" << src << "[b0 + oin[j]]);\n"
     << "  }\n"
     << "  for (; t < tend; ++t) {\n"
     << "    const unsigned long long base = tile_base(t);\n"
@@ -448,7 +473,7 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
     << "      if (tn < tend) {\n"
     << "        const unsigned long long bn = tile_base(tn) + gin;\n"
     << "#pragma unroll\n"
-    << "        for (int j = 0; j < 16; ++j) pre[j] = ldnt(&psi[bn + oin[j]]);\n"
+    << "        for (int j = 0; j < 16; ++j) pre[j] = ldnt(&" << src << "[bn + oin[j]]);\n"
     << "      }\n"
     << "    }\n";
   for (int k = 0; k < K; ++k) {
@@ -483,17 +508,17 @@ void emit_pass(std::ostringstream& s, const QProgram& qp, size_t pi, bool c128) 
       for (int i = 0; i < NR; ++i) s << "      tile[tp ^ " << Q.regpos_out[i] << "u] = v[" << i << "];\n";
       s << "      __syncthreads();\n";
     } else {
-      s << "      const unsigned long long bo = base + gout;\n"
+      s << "      const unsigned long long bo = " << obase << " + gout;\n"
         << "#pragma unroll\n"
-        << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], v[j]);\n";
+        << "      for (int j = 0; j < 16; ++j) stnt(&" << dst << "[bo + oout[j]], v[j]);\n";
     }
     s << "    }\n";
   }
   if (!dout) {
     s << "    {\n"
-      << "      const unsigned long long bo = base + gout;\n"
+      << "      const unsigned long long bo = " << obase << " + gout;\n"
       << "#pragma unroll\n"
-      << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], tile[tslot ^ swz(" << threads << "u * j)]);\n"
+      << "      for (int j = 0; j < 16; ++j) stnt(&" << dst << "[bo + oout[j]], tile[tslot ^ swz(" << threads << "u * j)]);\n"
       << "      __syncthreads();\n"
       << "    }\n";
   }
@@ -749,19 +774,12 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
   const int wpe = c128 ? (qp.fixed_forms ? 4 : 3) : (qp.fixed_forms ? 6 : 4);
   s << "extern \"C\" __global__ __launch_bounds__(" << threads << ") __attribute__((amdgpu_waves_per_eu(" << wpe
     << ")))\n"
-    << "void " << pass_prefix(qp) << pi << "(C* __restrict__ psi, const QC R* cf0, unsigned long long ntiles) {\n";
+    << "void " << pass_prefix(qp) << pi << pass_params(P) << " {\n";
   if (lds) s << "  __shared__ C tile[" << xpad((1u << (T - 1)) - 1) + 1 << "];\n";
-  s << "  const unsigned tid = threadIdx.x;\n"
-    << "  auto tile_base = [&](unsigned long long t) {\n"
-    << "    unsigned long long b = 0;\n";
-  for (int r = 0, sh = 0; r < P.nruns; ++r) {
-    s << "    b |= ((t >> " << sh << ") & " << u64((uint64_t(1) << P.rlen[r]) - 1) << ") << " << int(P.rdst[r])
-      << ";\n";
-    sh += P.rlen[r];
-  }
-  s << "    return b;\n"
-    << "  };\n"
-    << "  const unsigned long long gin = " << thread_gofs(P.tbits, first.ord) << ";\n"
+  s << "  const unsigned tid = threadIdx.x;\n";
+  emit_tile_bases(s, P);
+  const std::string src = src_name(P), dst = dst_name(P), obase = out_base(P);
+  s << "  const unsigned long long gin = " << thread_gofs(P.tbits, first.ord) << ";\n"
     << "  const unsigned long long gout = " << thread_gofs(P.tbits_out, last.ord) << ";\n";
   auto arr = [&](const char* name, const std::vector<uint64_t>& v) {
     s << "  const unsigned long long " << name << "[16] = {";
@@ -785,7 +803,7 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
     << "      asm volatile(\"\" : \"+v\"(g));\n"
     << "      const unsigned long long b0 = base + g;\n"
     << "#pragma unroll\n"
-    << "      for (int j = 0; j < 16; ++j) v[j] = ldnt(&psi[b0 + oin[j]]);\n"
+    << "      for (int j = 0; j < 16; ++j) v[j] = ldnt(&" << src << "[b0 + oin[j]]);\n"
     << "    }\n";
   if (!din) emit_exchange(s, tile_side, gat[0], bin[0], T);
   for (int k = 0; k < K; ++k) {
@@ -805,9 +823,9 @@ bool emit_pass_split(std::ostringstream& s, const QProgram& qp, size_t pi, bool 
   s << "    {\n"
     << "      unsigned long long g = gout;\n"
     << "      asm volatile(\"\" : \"+v\"(g));\n"
-    << "      const unsigned long long bo = base + g;\n"
+    << "      const unsigned long long bo = " << obase << " + g;\n"
     << "#pragma unroll\n"
-    << "      for (int j = 0; j < 16; ++j) stnt(&psi[bo + oout[j]], v[j]);\n"
+    << "      for (int j = 0; j < 16; ++j) stnt(&" << dst << "[bo + oout[j]], v[j]);\n"
     << "    }\n"
     << "  }\n"
     << "}\n";
@@ -1452,12 +1470,16 @@ int build_modules(const QProgram& qp, const char* prefix, Gen gen, QJit& out) {
   out.mods = mods;
   out.fn.resize(np);
   out.ntiles.resize(np);
+  out.wsrc.resize(np);
+  out.wdst.resize(np);
   out.threads = 1u << (qp.tile_bits - kQRegBits);
   for (size_t g = 0; g < groups; ++g)
     for (size_t i = cut[g]; i < cut[g + 1]; ++i) {
       const std::string name = prefix + std::to_string(i);
       QSV_CHECK_HIP(hipModuleGetFunction(&out.fn[i], mods[g]->mod, name.c_str()));
       out.ntiles[i] = qp.passes[i].ntiles;
+      out.wsrc[i] = qp.passes[i].wsrc;
+      out.wdst[i] = qp.passes[i].wdst;
     }
   return QSV_OK;
 }
@@ -1477,6 +1499,7 @@ std::string qtile_jit_adjoint_source(const QProgram& qp, bool c128, size_t p0, s
 }
 
 int qtile_jit_adjoint_check(const QProgram& qp) {
+  if (qtile_uses_work(qp)) return fail(QSV_EINVAL, "adjoint passes run in place");
   for (size_t i = 0; i < qp.passes.size(); ++i) {
     const int rows = adj_pass_rows(qp, i);
     const unsigned threads = 1u << (qp.tile_bits - kQRegBits);
@@ -1503,13 +1526,21 @@ int qtile_jit_launch_adjoint(const QJit& j, size_t i, void* phi, void* lam, cons
   return QSV_OK;
 }
 
-int qtile_jit_launch(const QJit& j, size_t i, void* psi, const QOp* ops, hipStream_t st) {
+int qtile_jit_launch(const QJit& j, size_t i, void* psi, void* work, const QOp* ops, hipStream_t st) {
   if (i >= j.fn.size()) return fail(QSV_EINVAL, "tiled pass index out of range");
   uint64_t ntiles = j.ntiles[i];
   const uint64_t want = std::min(uint64_t(device_cus()) * kJitBlocksPerCU,
                                  std::max<uint64_t>(1, ntiles / kJitTilesPerBlock));
   const unsigned grid = unsigned(ntiles < want ? ntiles : want);
   const void* cf = ops;
+  if (j.wsrc[i] || j.wdst[i]) {
+    if (!work) return fail(QSV_EINVAL, "tiled pass needs the work buffer");
+    const void* pin = j.wsrc[i] ? work : psi;
+    void* pout = j.wdst[i] ? work : psi;
+    void* args[] = {&pin, &pout, &cf, &ntiles};
+    QSV_CHECK_HIP(hipModuleLaunchKernel(j.fn[i], grid, 1, 1, j.threads, 1, 1, 0, st, args, nullptr));
+    return QSV_OK;
+  }
   void* args[] = {&psi, &cf, &ntiles};
   QSV_CHECK_HIP(hipModuleLaunchKernel(j.fn[i], grid, 1, 1, j.threads, 1, 1, 0, st, args, nullptr));
   return QSV_OK;

@@ -25,6 +25,7 @@ struct QJit {
   std::vector<std::shared_ptr<QJitModule>> mods;
   std::vector<hipFunction_t> fn;  // per pass
   std::vector<uint64_t> ntiles;   // per pass
+  std::vector<uint8_t> wsrc, wdst;  // per pass: reads / writes the work buffer
   unsigned threads = kQThreads;   // block size: 2^(tile bits - register bits)
 };
 
@@ -51,8 +52,9 @@ std::string qtile_jit_source(const QProgram& qp, bool c128, size_t p0, size_t p1
 // pass kernels for the current device.
 int qtile_jit_build(const QProgram& qp, bool c128, QJit& out);
 
-// Launch pass i.  ops: the program's device QOp array.
-int qtile_jit_launch(const QJit& j, size_t i, void* psi, const QOp* ops, hipStream_t st);
+// Launch pass i.  ops: the program's device QOp array; work: a buffer of the
+// state's size, used by the passes of an out-of-place end (else unused).
+int qtile_jit_launch(const QJit& j, size_t i, void* psi, void* work, const QOp* ops, hipStream_t st);
 
 // Compile one source to a gfx950 code object (no device needed).
 int qtile_jit_compile(const std::string& src, std::vector<char>& code);

@@ -183,7 +183,8 @@ class Program:
         return {"ngates": inf.ngates, "nlaunch": inf.nlaunch, "nplain": inf.nplain,
                 "ndiag": inf.ndiag, "ndense": list(inf.ndense), "alg_bytes": inf.alg_bytes,
                 "flops": inf.flops, "tiled_phases": inf.ntiled_phases,
-                "tiled_relabel": inf.ntiled_relabel, "fp_lane_ops": inf.fp_lane_ops}
+                "tiled_relabel": inf.ntiled_relabel, "fp_lane_ops": inf.fp_lane_ops,
+                "work_bytes": inf.work_bytes}
 
     def launches(self):
         """Every launch of one run, in order: (qubits, diag, coefficients) with

@@ -184,20 +184,47 @@ def test_jit_load_store_routes(name, dt, form):
     assert_close(got, want, tol(dt), f"{name} {dt}")
 
 
+@pytest.mark.parametrize("oop", [pytest.param(True, id="oop"), pytest.param(False, id="restore")])
 @pytest.mark.parametrize("form", JIT_FORMS)
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 @pytest.mark.parametrize("n", [18, 20, 22])
-def test_jit_relabeling_passes(n, dt, form):
+def test_jit_relabeling_passes(n, dt, form, oop, monkeypatch):
     """Compiled passes that move qubits onto the line bits (and back by the
     end of the program): brickwork plus every gate kind, against the
-    oracle, with at least one relabeling pass in the schedule."""
+    oracle, with at least one relabeling pass in the schedule.  The schedule
+    ends out of place when it can (the last pass reads a work buffer that the
+    pass before it wrote, and writes the state with every qubit home), or,
+    with QSV_JIT_OUT_OF_PLACE=0, with a gate-free restore pass."""
+    if not oop:
+        monkeypatch.setenv("QSV_JIT_OUT_OF_PLACE", "0")
     circ = W.brickwork(n, 20, seed=30 + n) + every_kind(n, n) + W.brickwork(n, 6, seed=n)
     psi = random_state(n, 40 + n, np.complex128)
     want = oracle.apply(n, circ, psi)
     prog = E.Program(n, circ, fuse_k=form, dtype=dt)
     info = prog.info()
-    print(f"relabel n={n} {dt}: passes={info['nlaunch']} relabeling={info['tiled_relabel']}", flush=True)
+    print(f"relabel n={n} {dt}: passes={info['nlaunch']} relabeling={info['tiled_relabel']} "
+          f"work={info['work_bytes']:.0f}", flush=True)
     assert info["tiled_relabel"] > 0
+    state_bytes = (1 << n) * (16 if dt == "c128" else 8)
+    assert info["work_bytes"] in (0, state_bytes)
+    if not oop:
+        assert info["work_bytes"] == 0
     x = psi if dt == "c128" else psi.astype(np.complex64)
     got = run(prog, x)
     assert_close(got, want, tol(dt), f"relabel n={n} {dt}")
+
+
+def test_jit_out_of_place_end_is_used():
+    """Some relabeling schedules end out of place: the n=20 complex128
+    brickwork-and-every-kind circuit in fixed forms does, and its result
+    matches the oracle on two runs in a row."""
+    n = 20
+    circ = W.brickwork(n, 20, seed=30 + n) + every_kind(n, n) + W.brickwork(n, 6, seed=n)
+    prog = E.Program(n, circ, fuse_k=E.TILED_JIT_FIXED)
+    info = prog.info()
+    print(f"out-of-place end: passes={info['nlaunch']} work={info['work_bytes']:.0f}", flush=True)
+    assert info["work_bytes"] == (1 << n) * 16
+    psi = random_state(n, 77, np.complex128)
+    assert_close(run(prog, psi), oracle.apply(n, circ, psi), C128, "out-of-place end")
+    # the same program twice on one stream: the work buffer is reused
+    assert_close(run(prog, psi), oracle.apply(n, circ, psi), C128, "second run")
