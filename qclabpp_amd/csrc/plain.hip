@@ -43,11 +43,13 @@ struct M4 {
 // Pair sweeps with the partner 2^b >= 64 amplitudes away: lanes 32-63 of a
 // wave hold the amplitudes 2^b above those of lanes 0-31, so each load
 // instruction reads 32 |0>-side and 32 |1>-side amplitudes, and
-// v_permlane32_swap gives every lane both amplitudes of its pair.  Against
-// two loads per lane (one per side), over the rotations of an n=30 complex128
-// brickwork: 0.75-0.76 of HBM peak instead of 0.74.  NP pairs per lane pair:
-// 2 where the two sides lie 2^12..2^21 amplitudes apart (64 KiB..32 MiB of
-// complex128), 1 elsewhere, the faster of the two at most b on that state.
+// v_permlane32_swap gives every lane both amplitudes of its pair.  NP pairs
+// per lane pair: 1 or 2, whichever measured faster for the target bit on the
+// MI355X (2^30 complex128 amplitudes, every b from 6 to 29, two runs that
+// agreed within 1%); the choice follows the byte-address bit of the partner
+// offset, so complex64 uses the same table one amplitude bit higher.  Over
+// all b: 6.19 TB/s on average with the table against 5.98 for two loads per
+// lane, one per side.
 __device__ __forceinline__ void swap_halves(uint32_t v, uint32_t& lo, uint32_t& hi) {
   // lo: every lane gets v of lane & 31; hi: v of lane | 32
   const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
@@ -78,7 +80,13 @@ __device__ __forceinline__ void pair_across_halves(const C v, C& x, C& y) {
   }
 }
 
-inline int pair_np(int b) { return b >= 12 && b <= 21 ? 2 : 1; }
+inline int pair_np(int b, size_t amp_bytes) {
+  // byte-address bits of the partner offset where one pair per lane pair won
+  constexpr uint64_t one = (1ull << 10) | (1ull << 11) | (1ull << 13) | (1ull << 21) | (1ull << 28) |
+                           (1ull << 29) | (1ull << 32) | (1ull << 33);
+  const int byte_bit = b + (amp_bytes == 16 ? 4 : 3);
+  return byte_bit < 64 && (one >> byte_bit & 1) ? 1 : 2;
+}
 
 template <typename C, int NP>
 __global__ __launch_bounds__(TPB) void k_dense1(C* __restrict__ psi, uint64_t cnt, int b,
@@ -166,9 +174,9 @@ __global__ __launch_bounds__(TPB) void k_phase1(C* __restrict__ psi, uint64_t cn
 }
 
 // ---- apply4 (controlled): 2x2 on the control==cs half ------------------------
-// The pair layout of k_dense1 over the control half (0.73-0.74 of HBM peak
-// against 0.69 with two loads per lane, over neighbour-qubit CX gates at n=30
-// complex128, the control half read and written in place).
+// The pair layout of k_dense1 over the control half (0.74 of HBM peak against
+// 0.69 with two loads per lane, over the neighbour-qubit CX gates of an n=30
+// complex128 brickwork, the control half read and written in place).
 // blo < bhi are the two bit positions; a1 carries the control bit, s the
 // target.  Each wave takes 32*NP pairs; lanes whose pair is out of range
 // still take part in the swaps (both lanes of a pair are in or out together).
@@ -456,7 +464,7 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
         QSV_LAUNCH_BLOCKS((k_dense1_lanes<C, NP>), blocks_for(N, per), TPB, 0, st,
                            psi, N, b, m);
       } else {
-        if (pair_np(b) == 2)
+        if (pair_np(b, sizeof(C)) == 2)
           QSV_LAUNCH_BLOCKS((k_dense1<C, 2>), blocks_for(N / 2, TPB), TPB, 0, st, psi, N / 2, b, m);
         else
           QSV_LAUNCH_BLOCKS((k_dense1<C, 1>), blocks_for(N / 2, TPB / 2), TPB, 0, st, psi, N / 2, b, m);
@@ -497,7 +505,7 @@ int launch_plain(const KOp& op, void* p, int n, hipStream_t st, const PlainTunin
         // NP pairs per lane pair: 32*NP pairs per wave, TPB/64 waves per block
         const uint64_t a1 = uint64_t(op.cs) << bc;
         const int blo = std::min(bc, bt), bhi = std::max(bc, bt);
-        if (pair_np(bt) == 2)
+        if (pair_np(bt, sizeof(C)) == 2)
           QSV_LAUNCH_BLOCKS((k_cdense1<C, 2>), blocks_for(N / 4, TPB), TPB, 0, st, psi, N / 4, blo,
                             bhi, a1, uint64_t(1) << bt, m);
         else
