@@ -516,8 +516,10 @@ def batch_leg(args):
            "roofline": {"bound": "hbm", "achieved": b.info["alg_bytes"] / (ms / 1e3) / 1e9, "peak": PEAK_HBM,
                         "unit": "GB/s", "frac": b.info["alg_bytes"] / (ms / 1e3) / 1e9 / PEAK_HBM,
                         "alg_bytes_per_run": b.info["alg_bytes"],
-                        "note": "whole run (every launch of the graph): each tile pass reads and writes "
-                                "its circuit's state once, a first pass only writes it"}}
+                        "note": "whole run (every launch of the graph): a tile pass reads and writes the "
+                                "tiles it runs once, a first pass only writes them; with register-phase "
+                                "passes the states are zero-filled once and a pass runs only the tiles "
+                                "that can hold nonzero amplitudes"}}
     if not args.no_cpu:
         import oracle
         sample = [i for i in range(args.batch)][:16]

@@ -70,10 +70,11 @@ struct Batch {
   std::vector<QPhase> qphases;
   std::vector<QOp> qops;
   std::vector<QPassInfo> qpinfo;
+  std::vector<int32_t> large_phase_ids;  // circuits of the register-phase passes
   Level small;
   std::vector<Level> large;
   DevBuf d_circs, d_passes, d_ops, d_items, d_tickets, d_done, d_stamps;
-  DevBuf d_qpasses, d_qphases, d_qops, d_qpinfo;
+  DevBuf d_qpasses, d_qphases, d_qops, d_qpinfo, d_zero_ids;
   // graph
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -199,15 +200,45 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
   // large levels: pass k of every large circuit that has one
   int maxp = 0;
   for (int c : large_ids) maxp = std::max(maxp, npass[c]);
+  // Register-phase passes start from |0...0>: before pass k only the
+  // addresses whose set bits lie in `support` (the tile bits of passes
+  // 0..k-1) can hold nonzero amplitudes.  A fill kernel zeroes the states
+  // first, pass 0 computes only its tile 0 (the other tiles hold zeros
+  // before and after it), and a later pass skips each tile whose base has a
+  // bit outside the support: the tile holds zeros, and its gates (on tile
+  // bits, or diagonal) keep them.
+  std::vector<uint64_t> support(ncirc, 0);
   for (int k = 0; k < maxp; ++k) {
     Level L;
     L.item_begin = items.size();
     for (int c : large_ids) {
       if (k >= npass[c]) continue;
       const int pi = first_pass[c] + k;
-      const uint64_t nt = opts.phases ? qpasses[pi].ntiles : passes[pi].ntiles;
-      for (uint64_t t = 0; t < nt; ++t) items.push_back(TItem{pi, 0, t});
-      L.tmax = std::max(L.tmax, opts.phases ? int(kQTileBits) : passes[pi].t);
+      if (!opts.phases) {
+        for (uint64_t t = 0; t < passes[pi].ntiles; ++t) items.push_back(TItem{pi, 0, t});
+        L.tmax = std::max(L.tmax, passes[pi].t);
+        continue;
+      }
+      const QPass& P = qpasses[pi];
+      uint64_t tmask = 0;
+      for (int i = 0; i < kQTileBits; ++i) tmask |= uint64_t(1) << P.tbits[i];
+      const size_t before = items.size();
+      for (uint64_t t = 0; t < P.ntiles; ++t) {
+        if (k == 0) {
+          if (t == 0) items.push_back(TItem{pi, 0, t});
+          continue;
+        }
+        uint64_t base = 0;
+        for (int r = 0, sh = 0; r < P.nruns; ++r) {
+          base |= ((t >> sh) & ((uint64_t(1) << P.rlen[r]) - 1)) << P.rdst[r];
+          sh += P.rlen[r];
+        }
+        if (base & ~support[c]) continue;
+        items.push_back(TItem{pi, 0, t});
+      }
+      qpinfo[pi].nitems = int32_t(items.size() - before);
+      support[c] |= tmask;
+      L.tmax = std::max(L.tmax, int(kQTileBits));
     }
     L.nitems = items.size() - L.item_begin;
     large.push_back(L);
@@ -241,6 +272,8 @@ int Batch::build(int dtype_, int n_circ, const int* nqubits, const qsv_gate* gat
     QSV_TRY(upload(d_qphases, qphases));
     QSV_TRY(upload(d_qops, qops.empty() ? std::vector<QOp>(1) : qops));
     QSV_TRY(upload(d_qpinfo, qpinfo));
+    for (int c : large_ids) large_phase_ids.push_back(c);
+    QSV_TRY(upload(d_zero_ids, large_phase_ids));
   }
   return QSV_OK;
 }
@@ -287,6 +320,13 @@ int Batch::enqueue(void* psi, hipStream_t st) {
     a.grid = L.grid;
     return a;
   };
+  if (opts.phases && !large_phase_ids.empty()) {
+    const int32_t* ids = static_cast<const int32_t*>(d_zero_ids.p);
+    const auto* cs = static_cast<const TCircuit*>(d_circs.p);
+    const int nz = int(large_phase_ids.size());
+    QSV_TRY(dbl ? launch_zero_states<double>(psi, cs, ids, nz, 64, st)
+                : launch_zero_states<float>(psi, cs, ids, nz, 64, st));
+  }
   for (size_t k = 0; k < large.size(); ++k) {
     if (opts.phases)
       QSV_TRY(dbl ? launch_qtile_level<double>(qargs(large[k], int(1 + k)), st)
@@ -408,7 +448,11 @@ int qsv_batch_info(const qsv_batch* b, qsv_batch_info_t* info, uint64_t* offsets
     double bytes = 0;
     for (const TPass& P : B.passes)
       bytes += (P.init_zero ? 1.0 : 2.0) * std::ldexp(eb, B.nq[size_t(P.circuit)]);
-    for (const QPassInfo& q : B.qpinfo) bytes += (q.first ? 1.0 : 2.0) * std::ldexp(eb, B.nq[size_t(q.circuit)]);
+    // register-phase circuits: the zero fill writes each state once, then
+    // a first pass writes its tile 0 and a later one reads and writes the
+    // tiles that can hold nonzeros
+    for (int c : B.large_phase_ids) bytes += std::ldexp(eb, B.nq[size_t(c)]);
+    for (const QPassInfo& q : B.qpinfo) bytes += (q.first ? 1.0 : 2.0) * double(q.nitems) * std::ldexp(eb, kQTileBits);
     info->alg_bytes = bytes;
   }
   return QSV_OK;

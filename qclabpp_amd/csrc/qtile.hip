@@ -483,11 +483,34 @@ __global__ __launch_bounds__(kQThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       __syncthreads();
       if (tid == 0) {
         const unsigned int prev = atomicAdd(&done_tiles[pi.circuit], 1u);
-        if (uint64_t(prev) + 1 == pass.ntiles) stamps[pi.circuit] = wall_clock64();
+        if (int64_t(prev) + 1 == int64_t(pi.nitems)) stamps[pi.circuit] = wall_clock64();
       }
     }
     __syncthreads();  // the tile is free again before the next item
     it = itn;
+  }
+}
+
+// Zeros over the states of the listed circuits (blockIdx.y picks one), 16
+// bytes per store: the batch's register-phase circuits before their first
+// passes, which then only compute the tile that holds |0...0>.
+template <typename R>
+__global__ __launch_bounds__(256) void k_zero_states(typename CT<R>::type* __restrict__ psi,
+                                                     const TCircuit* __restrict__ circs,
+                                                     const int32_t* __restrict__ ids) {
+  using C = typename CT<R>::type;
+  const TCircuit c = circs[ids[blockIdx.y]];
+  C* state = psi + c.offset;
+  constexpr int per = 16 / int(sizeof(C));  // amplitudes per 16-byte store
+  const uint64_t nvec = (uint64_t(1) << c.n) / per;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nvec;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    if constexpr (per == 1) {
+      st_stream(&state[i], mk<C>(0, 0));
+    } else {
+      st_stream(&state[2 * i], mk<C>(0, 0));
+      st_stream(&state[2 * i + 1], mk<C>(0, 0));
+    }
   }
 }
 
@@ -539,5 +562,19 @@ int launch_qtile_level(const QTileLevelArgs& a, hipStream_t st) {
 
 template int launch_qtile_level<double>(const QTileLevelArgs&, hipStream_t);
 template int launch_qtile_level<float>(const QTileLevelArgs&, hipStream_t);
+
+template <typename R>
+int launch_zero_states(void* psi, const TCircuit* circs, const int32_t* ids, int nids, unsigned blocks_per,
+                       hipStream_t st) {
+  using C = typename CT<R>::type;
+  if (nids <= 0) return QSV_OK;
+  hipLaunchKernelGGL((k_zero_states<R>), dim3(blocks_per, unsigned(nids)), dim3(256), 0, st, static_cast<C*>(psi),
+                     circs, ids);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+template int launch_zero_states<double>(void*, const TCircuit*, const int32_t*, int, unsigned, hipStream_t);
+template int launch_zero_states<float>(void*, const TCircuit*, const int32_t*, int, unsigned, hipStream_t);
 
 }  // namespace qsv

@@ -31,7 +31,7 @@ struct QPassInfo {
   int32_t circuit;  // index into the batch's circuits
   int32_t first;    // the circuit's first pass: its tiles start as |0...0>
   int32_t last;     // the circuit's last pass: completion stamping
-  int32_t pad;
+  int32_t nitems;   // work items of the pass (tiles that can hold nonzeros)
 };
 
 // One dependency level of register-phase passes (k_qtile_level, qtile.hip).
@@ -52,5 +52,11 @@ struct QTileLevelArgs {
 
 template <typename R>
 int launch_qtile_level(const QTileLevelArgs& a, hipStream_t st);
+
+// Zeros over the states of circuits ids[0..nids) (device array), blocks_per
+// blocks of 256 threads per circuit.
+template <typename R>
+int launch_zero_states(void* psi, const TCircuit* circs, const int32_t* ids, int nids, unsigned blocks_per,
+                       hipStream_t st);
 
 }  // namespace qsv
