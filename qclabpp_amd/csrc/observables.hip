@@ -164,7 +164,154 @@ __global__ __launch_bounds__(256) void k_pauli_apply(const C* __restrict__ in, C
   out[x] = mk<C>(static_cast<decltype(C::x)>(acc.x), static_cast<decltype(C::x)>(acc.y));
 }
 
+// Pauli sums whose terms flip at most one bit (diagonal terms, single X or
+// Y factors times diagonal ones: the transverse-field Ising Hamiltonian),
+// in a few passes over tiles of 2^12 amplitudes instead of one gather of
+// the state per flip mask.  Pass 0 takes the tiles of address bits 0..11;
+// pass j >= 1 the line bits 0..L-1 (L = 3 for complex128, 4 for complex64)
+// and the bits lo..lo+11-L.  A pass handles the flips of the bits [flo, fhi)
+// within its tile (pass 0 also the diagonal terms): each block loads its
+// tile of `in` into LDS, where the partner in[x ^ 2^b] of every such flip
+// bit b is, and writes out[x] (pass 0) or adds to it (later passes).
+constexpr int kFlipTileBits = 12;
+constexpr int kFlipThreads = 512;  // 8 amplitudes per thread, 16 waves per CU (two 64 KiB tiles)
+constexpr int kFlipGroupsMax = 32;  // groups per launch
+typedef const __attribute__((address_space(4))) DevPauli* DevPauliConst;
+// The groups one launch handles: group index and the tile-local bit of its
+// flip (-1: the diagonal group).
+struct FlipGroups {
+  int32_t n;
+  int16_t g[kFlipGroupsMax];
+  int8_t lb[kFlipGroupsMax];
+};
+template <typename C>
+__global__ __launch_bounds__(kFlipThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_pauli_flip_tiles(
+    const C* __restrict__ in, C* __restrict__ out, DevPauliConst terms, const int* __restrict__ gstart,
+    FlipGroups fg, int line, int lo, int first, uint64_t blk0) {
+  constexpr int TN = 1 << kFlipTileBits;
+  constexpr int PER = TN / kFlipThreads;
+  __shared__ C tile[TN];
+  // the tile's base address: tile index bits below lo go to address bits
+  // line..lo-1, the rest above the tile's bits
+  const int hi = lo + kFlipTileBits - line;  // first address bit above the tile
+  const uint64_t t = blk0 + blockIdx.x;
+  const uint64_t low_n = uint64_t(lo - line);
+  const uint64_t base = ((t & ((uint64_t(1) << low_n) - 1)) << line) | ((t >> low_n) << hi);
+  const uint32_t lmask = (1u << line) - 1;
+  auto addr = [&](uint32_t p) { return base | (p & lmask) | (uint64_t(p >> line) << lo); };
+  // the tile, and the partial sums of earlier launches
+  double2 acc[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t p = threadIdx.x + uint32_t(kFlipThreads) * j;
+    tile[p] = in[addr(p)];
+    acc[j] = first ? make_double2(0, 0) : to_d(out[addr(p)]);
+  }
+  __syncthreads();
+  // groups outside, the thread's amplitudes inside: each term's scalar
+  // loads serve all of them
+  for (int i = 0; i < fg.n; ++i) {
+    const int g = fg.g[i], lb = fg.lb[i];
+    const int k0 = gstart[g], k1 = gstart[g + 1];
+    const uint64_t f = terms[k0].flip;
+    double cr[PER], ci[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) cr[j] = ci[j] = 0;
+    for (int k = k0; k < k1; ++k) {
+      const uint64_t zm = terms[k].zmask, pm = terms[k].pmask, pv = terms[k].pval;
+      const double tr = terms[k].c_re, tim = terms[k].c_im;
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        const uint64_t y = addr(threadIdx.x + uint32_t(kFlipThreads) * j) ^ f;
+        const double on = (y & pm) == pv ? ((__popcll(y & zm) & 1) ? -1.0 : 1.0) : 0.0;
+        cr[j] += on * tr;
+        ci[j] += on * tim;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t p = threadIdx.x + uint32_t(kFlipThreads) * j;
+      const double2 v = to_d(tile[lb < 0 ? p : p ^ (1u << lb)]);
+      acc[j].x += cr[j] * v.x - ci[j] * v.y;
+      acc[j].y += cr[j] * v.y + ci[j] * v.x;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    out[addr(threadIdx.x + uint32_t(kFlipThreads) * j)] =
+        mk<C>(static_cast<decltype(C::x)>(acc[j].x), static_cast<decltype(C::x)>(acc[j].y));
+}
+
 }  // namespace
+
+bool pauli_flips_single(const std::vector<DevPauli>& terms, const std::vector<int>& group_start) {
+  for (size_t g = 0; g + 1 < group_start.size(); ++g)
+    if (__builtin_popcountll(terms[size_t(group_start[g])].flip) > 1) return false;
+  return true;
+}
+
+template <typename R>
+int launch_pauli_apply_tiles(const void* in, void* out, int n, const std::vector<DevPauli>& host_terms,
+                             const std::vector<int>& host_group_start, const void* dev_terms,
+                             const void* dev_group_start, hipStream_t st) {
+  using C = typename CT<R>::type;
+  if (n < kFlipTileBits) return fail(QSV_EINVAL, "tiled Pauli sum needs at least 12 qubits");
+  if (host_terms.size() > size_t(kMaxPauliTerms)) return fail(QSV_EINVAL, "Pauli sum has too many terms");
+  if (!pauli_flips_single(host_terms, host_group_start))
+    return fail(QSV_EINVAL, "tiled Pauli sum: a term flips more than one qubit");
+  const int line = sizeof(C) == 16 ? 3 : 4;
+  const int ngroups = int(host_group_start.size()) - 1;
+  const uint64_t ntiles = uint64_t(1) << (n - kFlipTileBits);
+  const auto* ti = static_cast<const C*>(in);
+  auto* to = static_cast<C*>(out);
+  const auto dt = (DevPauliConst)dev_terms;
+  const auto* dg = static_cast<const int*>(dev_group_start);
+  bool first = true;
+  // one tile range: launches of at most kFlipGroupsMax of its groups each,
+  // the first of the sum writing `out`, the others adding to it
+  auto run = [&](int lo, int flo, int fhi, bool diag) -> int {
+    FlipGroups fg{};
+    auto flush = [&]() -> int {
+      if (fg.n == 0) return QSV_OK;
+      QSV_LAUNCH_BLOCKS((k_pauli_flip_tiles<C>), ntiles, kFlipThreads, 0, st, ti, to, dt, dg, fg, line, lo,
+                        first ? 1 : 0);
+      QSV_CHECK_HIP(hipGetLastError());
+      first = false;
+      fg.n = 0;
+      return QSV_OK;
+    };
+    for (int g = 0; g < ngroups; ++g) {
+      const uint64_t f = host_terms[size_t(host_group_start[g])].flip;
+      int lb;
+      if (f == 0) {
+        if (!diag) continue;
+        lb = -1;
+      } else {
+        const int b = __builtin_ctzll(f);
+        if (b < flo || b >= fhi) continue;
+        lb = b < line ? b : b - lo + line;
+      }
+      fg.g[fg.n] = int16_t(g);
+      fg.lb[fg.n] = int8_t(lb);
+      if (++fg.n == kFlipGroupsMax) QSV_TRY(flush());
+    }
+    return flush();
+  };
+  // the tiles of address bits 0..11: their flips and the diagonal terms
+  QSV_TRY(run(line, 0, kFlipTileBits, true));
+  // then the flips of the next 12 - line bits each, in tiles of the line bits
+  // and a 12 - line bit range holding them (slid down onto the top bits for
+  // the last ones)
+  const int step = kFlipTileBits - line;
+  for (int f0 = kFlipTileBits; f0 < n; f0 += step)
+    QSV_TRY(run(std::min(f0, n - step), f0, std::min(n, f0 + step), false));
+  if (first) {  // no term at all reached a launch: out = 0
+    FlipGroups none{};
+    QSV_LAUNCH_BLOCKS((k_pauli_flip_tiles<C>), ntiles, kFlipThreads, 0, st, ti, to, dt, dg, none, line, line, 1);
+    QSV_CHECK_HIP(hipGetLastError());
+  }
+  return QSV_OK;
+}
 
 void group_terms(const std::vector<PauliOp>& terms, std::vector<DevPauli>& out,
                  std::vector<int>& group_start) {
@@ -238,6 +385,10 @@ template int launch_pauli_inner<float>(const void*, const void*, int, const Paul
 template int launch_diag_expect<double>(const void*, int, const std::vector<PauliOp>&, void*, void*, hipStream_t);
 template int launch_diag_expect<float>(const void*, int, const std::vector<PauliOp>&, void*, void*, hipStream_t);
 template int launch_pauli_apply<double>(const void*, void*, int, const void*, const void*, int, int, hipStream_t);
+template int launch_pauli_apply_tiles<double>(const void*, void*, int, const std::vector<DevPauli>&,
+                                              const std::vector<int>&, const void*, const void*, hipStream_t);
+template int launch_pauli_apply_tiles<float>(const void*, void*, int, const std::vector<DevPauli>&,
+                                             const std::vector<int>&, const void*, const void*, hipStream_t);
 template int launch_pauli_apply<float>(const void*, void*, int, const void*, const void*, int, int, hipStream_t);
 
 }  // namespace qsv

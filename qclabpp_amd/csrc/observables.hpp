@@ -54,6 +54,16 @@ struct DevPauli {
   uint64_t flip, zmask, pmask, pval;
   double c_re, c_im;  // coeff * phase
 };
+
+// The same for sums whose terms each flip at most one bit (n >= 12): a few
+// passes over 2^12-amplitude tiles, each holding the partners of the flips
+// it handles, instead of one gather of the state per flip mask.
+template <typename R>
+int launch_pauli_apply_tiles(const void* in, void* out, int n, const std::vector<DevPauli>& host_terms,
+                             const std::vector<int>& host_group_start, const void* dev_terms,
+                             const void* dev_group_start, hipStream_t st);
+// Whether every group (terms sharing a flip mask) flips at most one bit.
+bool pauli_flips_single(const std::vector<DevPauli>& terms, const std::vector<int>& group_start);
 void group_terms(const std::vector<PauliOp>& terms, std::vector<DevPauli>& out,
                  std::vector<int>& group_start);
 

@@ -49,6 +49,48 @@ def test_apply_pauli_sum_vs_matrix():
     assert np.abs(out.cpu().numpy() - H.matrix(n) @ psi).max() < 1e-14
 
 
+def single_flip_H(n):
+    """Terms that each flip at most one qubit (the tiled H|psi> path): the
+    TFIM chain plus Y factors, Z strings and X on the last qubit."""
+    H = O.tfim(n)
+    H.add(0.3, [(1, "Y"), (n - 1, "Z")]).add(-1.1, [(0, "Y")]).add(0.4, [(n - 1, "X"), (0, "Z")])
+    H.add(0.25, [(0, "Z"), (2, "Z"), (4, "Z")]).add(0.6, [(n // 2, "Y"), (n // 2 + 1, "Z")])
+    return H
+
+
+def oracle_apply_sum(n, H, psi):
+    """sum_t c_t P_t psi with each Pauli factor applied as the oracle's gate."""
+    ctor = {"X": G.PauliX, "Y": G.PauliY, "Z": G.PauliZ}
+    out = np.zeros_like(psi)
+    for c, fs in H.terms:
+        out += c * oracle.apply(n, [ctor[p](q) for q, p in fs], psi)
+    return out
+
+
+@pytest.mark.parametrize("tiles", ["1", "0"])
+@pytest.mark.parametrize("dt,tol", [("c128", 1e-13), ("c64", 2e-5)])
+@pytest.mark.parametrize("n,kind", [(12, "single"), (14, "single"), (22, "single"), (14, "mixed")])
+def test_apply_pauli_sum_vs_oracle(n, kind, dt, tol, tiles, monkeypatch):
+    """H|psi> at 12 to 22 qubits against the oracle's gates.  With
+    QSV_PAULI_TILES=1, sums whose terms flip at most one qubit take the tiled
+    passes (one to three of them, every flip bit in exactly one), which
+    otherwise serve states above 1 GiB; other sums, and every sum with =0,
+    take the per-flip-mask gather."""
+    import torch
+    monkeypatch.setenv("QSV_PAULI_TILES", tiles)
+    print(f"apply pauli sum n={n} {kind} {dt} tiles={tiles}", flush=True)
+    H = single_flip_H(n) if kind == "single" else mixed_H(n)
+    psi = random_state(n, 11 + n, np.complex128)
+    want = oracle_apply_sum(n, H, psi)
+    cdt = np.complex128 if dt == "c128" else np.complex64
+    out = torch.full((1 << n,), float("nan"), dtype=torch.complex128 if dt == "c128" else torch.complex64,
+                     device="cuda")
+    O.apply_pauli_sum(dev(psi.astype(cdt)), out, H)
+    got = out.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - want).max() / np.abs(want).max() < tol
+
+
 def test_inner_and_norm():
     n = 12
     a, b = random_state(n, 1), random_state(n, 2)
