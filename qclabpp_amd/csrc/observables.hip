@@ -106,6 +106,12 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_rows(const double2* __re
 constexpr int kDiagMax = 32;
 
 // Up to 32 diagonal Pauli terms in one pass; partials[t * kRedBlocks + block].
+// The grid's stride is 2^sb (sb = 18), so the amplitudes x = x0 + k 2^sb of a
+// thread share their low sb bits: a term's sign is (-1)^popc(x0 & z), fixed
+// for the thread, times (-1)^popc(k & (z >> sb)).  Terms with no bit at or
+// above sb share one running sum; the others accumulate with the second
+// factor only, and the first is applied at the end.  Negation is exact, so
+// the sums are bit-identical to applying the full sign per amplitude.
 template <typename C>
 __global__ __launch_bounds__(kRedThreads) void k_diag_expect(const C* __restrict__ psi, uint64_t N,
                                                              const uint64_t* __restrict__ zm,
@@ -114,20 +120,28 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_expect(const C* __restrict
   __shared__ uint64_t z[kDiagMax];
   if (threadIdx.x < nt) z[threadIdx.x] = zm[threadIdx.x];
   __syncthreads();
+  const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;  // a power of two
+  const int sb = __ffsll(static_cast<long long>(stride)) - 1;
+  const uint64_t x0 = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x;
+  double all = 0;
   double acc[kDiagMax];
 #pragma unroll
   for (int t = 0; t < kDiagMax; ++t) acc[t] = 0;
-  const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
-  for (uint64_t x = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; x < N; x += stride) {
+  uint64_t k = 0;
+  for (uint64_t x = x0; x < N; x += stride, ++k) {
     const double2 v = to_d(psi[x]);
     const double p = v.x * v.x + v.y * v.y;
+    all += p;
 #pragma unroll
-    for (int t = 0; t < kDiagMax; ++t)
-      if (t < nt) acc[t] += (__popcll(x & z[t]) & 1) ? -p : p;
+    for (int t = 0; t < kDiagMax; ++t) {
+      const uint64_t zh = t < nt ? z[t] >> sb : 0;
+      if (zh) acc[t] += (__popcll(k & zh) & 1) ? -p : p;
+    }
   }
   for (int t = 0; t < nt; ++t) {
-    const double2 s = block_sum(make_double2(acc[t], 0), sh);
-    if (threadIdx.x == 0) partials[size_t(t) * kRedBlocks + blockIdx.x] = s;
+    const double s = (z[t] >> sb) ? acc[t] : all;
+    const double2 r = block_sum(make_double2((__popcll(x0 & z[t]) & 1) ? -s : s, 0), sh);
+    if (threadIdx.x == 0) partials[size_t(t) * kRedBlocks + blockIdx.x] = r;
     __syncthreads();
   }
 }

@@ -28,8 +28,11 @@ def mixed_H(n):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-13), (np.complex64, 2e-5)])
-@pytest.mark.parametrize("n", [6, 14])
+@pytest.mark.parametrize("n", [6, 14, 20])
 def test_expectation_vs_oracle(n, dtype, tol):
+    """Per-term <psi|P|psi> against the oracle; at n=20 the diagonal terms on
+    qubits 0 and 1 (address bits 18 and 19) vary along each thread's
+    amplitudes, the others do not."""
     print(f"expectation n={n} {dtype.__name__}", flush=True)
     psi = random_state(n, 7, np.complex128)
     H = mixed_H(n)
