@@ -537,7 +537,8 @@ def batch_leg(args):
 
 def precision_leg(args):
     """complex64 against complex128 on the seeded depth-20 brickwork (seed
-    20261203), both run with the compiled tiled passes from |0...0>.  At n=28 the
+    20261203), both run with the fixed-form compiled tiled passes (the
+    fastest form) from |0...0>.  At n=28 the
     complex64 state is compared with the complex128 one: max |delta amp|,
     norm drift |<psi|psi> - 1| and the error of <Z_i Z_{i+1}> (i = 0..n-2)
     and of their sum, every reduction accumulated in FP64.  At n=33 the
@@ -560,7 +561,7 @@ def precision_leg(args):
         res = {}
         states = {}
         for dt, tdt in (("c128", torch.complex128), ("c64", torch.complex64)):
-            prog = E.Program(n, circ, fuse_k=E.TILED_JIT, dtype=dt)
+            prog = E.Program(n, circ, fuse_k=E.TILED_JIT_FIXED, dtype=dt)
             psi = E.new_state(n, tdt)
             ms = timed(torch, lambda: prog.run(psi), 1, stream)
             vals, tot = O.expectation(psi, zz)
@@ -570,7 +571,7 @@ def precision_leg(args):
         del states
         torch.cuda.empty_cache()
         out["n28"] = {
-            "workload": f"brickwork n={n} depth=20 ({len(circ)} gates), compiled tiled passes",
+            "workload": f"brickwork n={n} depth=20 ({len(circ)} gates), fixed-form compiled tiled passes",
             "max_abs_amp_c64_vs_c128": d,
             "norm_drift_c64": res["c64"]["norm_drift"], "norm_drift_c128": res["c128"]["norm_drift"],
             "max_abs_zz_err_c64": float(np.abs(res["c64"]["zz"] - res["c128"]["zz"]).max()),
@@ -580,20 +581,20 @@ def precision_leg(args):
     if args.precision_big_n > 0:
         n = args.precision_big_n
         circ = W.brickwork(n, 20, seed=W.SEED_PREC)
-        prog = E.Program(n, circ, fuse_k=E.TILED_JIT, dtype="c64")
+        prog = E.Program(n, circ, fuse_k=E.TILED_JIT_FIXED, dtype="c64")
         psi = E.new_state(n, torch.complex64)
         ms = timed(torch, lambda: prog.run(psi), 1, stream)
         vals, tot = O.expectation(psi, O.zz_chain(n))
         info = prog.info()
         out["n33"] = {
-            "workload": f"brickwork n={n} depth=20 ({len(circ)} gates), complex64, compiled tiled passes",
+            "workload": f"brickwork n={n} depth=20 ({len(circ)} gates), complex64, fixed-form compiled tiled passes",
             "state_bytes": (1 << n) * 8, "ms": ms, "gate_passes_per_s": len(circ) / (ms / 1e3),
             "hbm_GBps": info["alg_bytes"] / (ms / 1e3) / 1e9, "launches": info["nlaunch"],
             "norm_drift_fp64": abs(O.norm2(psi) - 1.0),
             "zz": [float(v) for v in vals], "zz_sum": tot}
         del prog
         if args.precision_big_c128:
-            p128 = E.Program(n, circ, fuse_k=E.TILED_JIT, dtype="c128")
+            p128 = E.Program(n, circ, fuse_k=E.TILED_JIT_FIXED, dtype="c128")
             ref = E.new_state(n, torch.complex128)
             ms128 = timed(torch, lambda: p128.run(ref), 1, stream)
             v128, t128 = O.expectation(ref, O.zz_chain(n))
