@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -242,34 +243,47 @@ int adjoint_schedule(int n, int dtype, const std::vector<qsv_gate>& gs, QProgram
   return QSV_OK;
 }
 
-int adjoint_reverse_jit(void* psi, void* lam, int n, int dtype, const std::vector<qsv_gate>& gs,
-                        double2* partials, int row0, std::vector<int>& row_param,
-                        std::vector<double>& row_scale, hipStream_t st, bool& done,
-                        int64_t& launches) {
-  done = false;
+// The host half of the compiled reverse sweep (schedule, op table and
+// modules), built on a worker thread while the forward pass and H|phi>
+// run on the device.  ok stays false when a pass holds more generator rows
+// than LDS does (the caller then runs the interpreted sweep).
+struct AdjReversePlan {
   QProgram qp;
-  std::vector<int> pass_row0;
-  QSV_TRY(adjoint_schedule(n, dtype, gs, qp, pass_row0, row_param, row_scale));
-  if (qtile_jit_adjoint_check(qp) != QSV_OK) {
-    // a pass with more generator rows than LDS holds: the caller falls back
-    row_param.clear();
-    row_scale.clear();
+  std::vector<int> pass_row0, row_param;
+  std::vector<double> row_scale;
+  QJit qj;
+  bool ok = false;
+  int rc = QSV_OK;
+  std::string err;
+};
+
+int adjoint_reverse_prepare(int dev, int n, int dtype, const std::vector<qsv_gate>& gs,
+                            AdjReversePlan& plan) {
+  QSV_CHECK_HIP(hipSetDevice(dev));  // module loads go to the caller's device
+  QSV_TRY(adjoint_schedule(n, dtype, gs, plan.qp, plan.pass_row0, plan.row_param, plan.row_scale));
+  if (qtile_jit_adjoint_check(plan.qp) != QSV_OK) {
+    plan.row_param.clear();
+    plan.row_scale.clear();
     return QSV_OK;
   }
-  QJit qj;
-  QSV_TRY(qtile_jit_build_adjoint(qp, dtype == QSV_C128, qj));
+  QSV_TRY(qtile_jit_build_adjoint(plan.qp, dtype == QSV_C128, plan.qj));
+  plan.ok = true;
+  return QSV_OK;
+}
+
+int adjoint_reverse_jit(void* psi, void* lam, const AdjReversePlan& plan, double2* partials, int row0,
+                        hipStream_t st, int64_t& launches) {
   DevArray<QOp> d_ops;
-  QSV_TRY(d_ops.upload(qp.ops));
+  QSV_TRY(d_ops.upload(plan.qp.ops));
   QSV_TRY(g_adjprof.record(2, st));
-  for (size_t i = 0; i < qp.passes.size(); ++i) {
-    double* part = reinterpret_cast<double*>(partials + size_t(row0 + pass_row0[i]) * kRedBlocks);
-    QSV_TRY(qtile_jit_launch_adjoint(qj, i, psi, lam, d_ops.p, part, st));
+  for (size_t i = 0; i < plan.qp.passes.size(); ++i) {
+    double* part = reinterpret_cast<double*>(partials + size_t(row0 + plan.pass_row0[i]) * kRedBlocks);
+    QSV_TRY(qtile_jit_launch_adjoint(plan.qj, i, psi, lam, d_ops.p, part, st));
   }
   QSV_TRY(g_adjprof.record(3, st));
-  launches = int64_t(qp.passes.size());
+  launches = int64_t(plan.qp.passes.size());
   // the op table is freed with d_ops: finish before returning
   QSV_CHECK_HIP(hipStreamSynchronize(st));
-  done = true;
   return QSV_OK;
 }
 
@@ -538,6 +552,19 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
         return fail(QSV_EUNSUP, "gate kind " + std::to_string(g.kind) + " is not differentiable");
     }
   }
+  const int form = adjoint_form(n);
+  if (form < 0) return fail(QSV_EUNSUP, "QSV_ADJOINT=jit: compiled passes need hiprtc and n >= 12");
+  // the compiled reverse sweep's host work overlaps everything up to its launch
+  AdjReversePlan plan;
+  std::future<void> prep;
+  if (form == 1) {
+    int dev = 0;
+    QSV_CHECK_HIP(hipGetDevice(&dev));
+    prep = std::async(std::launch::async, [&plan, &gs, dev, n, dtype] {
+      plan.rc = adjoint_reverse_prepare(dev, n, dtype, gs, plan);
+      if (plan.rc != QSV_OK) plan.err = get_error();
+    });
+  }
   // forward: phi = U psi0
   Program fwd;
   QSV_TRY(fwd.build(n, dtype, gs.data(), gs.size(), fuse_k, 0));
@@ -562,17 +589,15 @@ int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_g
   auto* red = partials + nrows * kRedBlocks;
   // energy row: <phi|lam>
   QSV_TRY(inner_any(dtype, psi, lam, n, PauliOp(), partials, st));
-  const int form = adjoint_form(n);
-  if (form < 0) return fail(QSV_EUNSUP, "QSV_ADJOINT=jit: compiled passes need hiprtc and n >= 12");
   if (form == 1) {
-    std::vector<int> rp;
-    std::vector<double> rs;
-    bool done = false;
-    QSV_TRY(adjoint_reverse_jit(psi, lam, n, dtype, gs, partials, 1, rp, rs, st, done,
-                                g_adjprof.reverse_launches));
-    if (done) {
+    prep.get();
+    if (plan.rc != QSV_OK) return fail(plan.rc, plan.err);
+    if (plan.ok) {
+      QSV_TRY(adjoint_reverse_jit(psi, lam, plan, partials, 1, st, g_adjprof.reverse_launches));
       g_adjprof.compiled = 1;
       g_adjprof.valid = true;
+      const std::vector<int>& rp = plan.row_param;
+      const std::vector<double>& rs = plan.row_scale;
       const int rows = 1 + int(rp.size());
       QSV_TRY(launch_reduce_rows(partials, rows, red, st));
       std::vector<double2> host(rows);
