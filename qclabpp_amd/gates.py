@@ -100,8 +100,21 @@ def inverse(gates: Iterable[Gate]) -> list:
 def records(gates: Iterable[Gate], op: str | None = None, offset: int = 0):
     gl = list(gates)
     arr = (QsvGate * max(1, len(gl)))()
+    if not gl:
+        return arr, 0
+    # filled column by column through a numpy view of the records (the same
+    # fields Gate.record sets, without a ctypes object per gate)
+    v = np.ctypeslib.as_array(arr)
+    v["kind"] = [g.kind for g in gl]
+    v["q"][:, 0] = [g.qubits[0] + offset for g in gl]
+    v["q"][:, 1] = [(g.qubits[1] + offset) if len(g.qubits) > 1 else 0 for g in gl]
+    v["ctrl_state"] = [g.ctrl_state for g in gl]
+    v["op"] = [ord(g.op if op is None else op) for g in gl]
+    v["param"] = [g.param for g in gl]
+    v["theta"] = [float(g.theta) for g in gl]
     for i, g in enumerate(gl):
-        arr[i] = g.record(op, offset)
+        if g.cs is not None or g.mat is not None:
+            arr[i] = g.record(op, offset)
     return arr, len(gl)
 
 

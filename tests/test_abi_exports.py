@@ -61,3 +61,26 @@ def test_matrix_argument_checks_need_no_gpu():
     assert lib.qsv_host_circuit_matrix(None, 2, _lib.QSV_C128, arr, cnt, 0) == _lib.QSV_EINVAL
     out = ctypes.c_double()
     assert lib.qsv_circuit_nrmf(2, 9, arr, cnt, arr, cnt, 0, ctypes.byref(out)) == _lib.QSV_EDTYPE
+
+
+def test_records_match_per_gate_records():
+    """The column-wise record builder writes the same bytes as Gate.record,
+    for every op override and offset, including explicit (cos, sin) and
+    matrix gates."""
+    import numpy as np
+
+    from qclabpp_amd import gates as G
+    from qclabpp_amd import workloads as W
+
+    rng = np.random.default_rng(3)
+    u1 = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
+    mixed = [G.Hadamard(1), G.RotationY(2, 0.3, param=4), G.RotationZ(0, cos=0.6, sin=0.8),
+             G.QGate1(3, u1), G.CX(0, 2, controlState=0), G.CPhase(1, 3, 0.7, param=0), G.CZ(1, 2)]
+    for circ in (mixed, W.tfim_ansatz(12, 3), W.brickwork(10, 4)):
+        for op, off in ((None, 0), ("C", 3), ("T", 1)):
+            arr, cnt = G.records(circ, op, off)
+            assert cnt == len(circ)
+            for i, g in enumerate(circ):
+                assert bytes(arr[i]) == bytes(g.record(op, off))
+    arr, cnt = G.records([])
+    assert cnt == 0
