@@ -287,7 +287,10 @@ int qsv_dev_apply_pauli_sum(const void* in, void* out, int n, int dtype,
  * The reverse sweep runs as passes compiled for the circuit's structure
  * over both states from 12 qubits up where hiprtc loads (the schedule and
  * kernels cached like QSV_TILED_JIT programs), else as interpreted LDS
- * tile passes; QSV_ADJOINT=interp|jit in the environment forces one.      */
+ * tile passes; QSV_ADJOINT=interp|jit in the environment forces one.
+ * The compiled sweep's schedule and kernels are prepared on one extra host
+ * thread, started and joined within the call, while the forward pass and
+ * H|phi> run on `stream`; the call returns after the stream is idle.      */
 int qsv_dev_adjoint_gradient(void* psi, void* lam, int n, int dtype, const qsv_gate* gates,
                              size_t ngates, const double* theta, size_t nparams,
                              const qsv_pauli_term* terms, size_t nterms, int fuse_k,
